@@ -1,0 +1,73 @@
+"""Builds the native libraries of amuse_amd in-tree with hipcc (gfx950 only).
+
+    python -m amuse_amd.build          # libg6.so
+    python -m amuse_amd.build --isa    # also dump the gfx950 assembly to _build/
+
+The libraries land in amuse_amd/lib/ so they travel with the source tree; no
+JIT cache and no site-packages install is involved.
+"""
+from __future__ import annotations
+
+import os
+import shutil
+import subprocess
+import sys
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+ROOT = os.path.dirname(HERE)
+CSRC = os.path.join(HERE, "csrc")
+LIBDIR = os.path.join(HERE, "lib")
+ARCH = "gfx950"
+
+# Packed FP32 instructions are kept out of every code object (the compiler
+# would otherwise form v_pk_* from float2-shaped code).
+DEVICE_FLAGS = ["-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops"]
+
+LIBS = {
+    "libg6.so": ["g6_kernels.hip", "g6_abi.hip"],
+}
+
+
+def hipcc() -> str:
+    for cand in (os.environ.get("HIPCC"), "/opt/rocm/bin/hipcc", shutil.which("hipcc")):
+        if cand and os.path.exists(cand):
+            return cand
+    raise RuntimeError("hipcc not found")
+
+
+def _run(cmd: list[str]) -> None:
+    res = subprocess.run(cmd, capture_output=True, text=True)
+    if res.returncode != 0:
+        sys.stderr.write(res.stdout + res.stderr)
+        raise RuntimeError("build failed: " + " ".join(cmd))
+
+
+def build_lib(name: str, isa: bool = False) -> str:
+    os.makedirs(LIBDIR, exist_ok=True)
+    srcs = [os.path.join(CSRC, s) for s in LIBS[name]]
+    out = os.path.join(LIBDIR, name)
+    tmp = out + ".tmp"
+    cmd = [hipcc(), "-O3", "-std=c++17", f"--offload-arch={ARCH}", "-fPIC", "-shared",
+           "-fvisibility=hidden", "-Wno-unused-command-line-argument",
+           *DEVICE_FLAGS, "-I", os.path.join(ROOT, "include"), *srcs,
+           "-Wl,-Bsymbolic", "-o", tmp]
+    _run(cmd)
+    os.replace(tmp, out)
+    if isa:
+        bdir = os.path.join(ROOT, "_build")
+        os.makedirs(bdir, exist_ok=True)
+        for s in srcs:
+            base = os.path.splitext(os.path.basename(s))[0]
+            _run([hipcc(), "-O3", "-std=c++17", f"--offload-arch={ARCH}", "--cuda-device-only",
+                  "-S", *DEVICE_FLAGS, "-I", os.path.join(ROOT, "include"), s,
+                  "-o", os.path.join(bdir, base + ".s")])
+    return out
+
+
+def build_all(isa: bool = False) -> list[str]:
+    return [build_lib(n, isa) for n in LIBS]
+
+
+if __name__ == "__main__":
+    for p in build_all(isa="--isa" in sys.argv):
+        print("built", p)

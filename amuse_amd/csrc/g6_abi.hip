@@ -1,0 +1,926 @@
+// GRAPE-6 C ABI on MI355X: the unit state machine behind g6_open ...
+// g6calc_lasthalf2 (reference: lib/g6lib/g6lib.c:190-481 for semantics,
+// lib/sapporo_light/sapporo.cpp:19-272 for the firsthalf/lasthalf split).
+//
+// One global unit per process, like both reference libraries; not
+// thread-safe, like them.  All device work goes to one private stream:
+//
+//   set_j_particle   host staging (deduplicated per slot, last write wins)
+//   firsthalf        staging -> pinned -> HBM + scatter kernel; predictor
+//                    pass if ti moved; i-block -> pinned -> HBM; with
+//                    G6_EAGER=1 the force kernel is launched here too
+//   lasthalf[2]      force kernel (+ nn if lasthalf2) + chunk reduction,
+//                    one D2H of the packed results, stream sync, unpack.
+//
+// Errors are returned as negative codes (g6x.h); the library never exits.
+#include "g6_kernels.h"
+#include "../../include/g6lib.h"
+#include "../../include/g6x.h"
+
+#include <algorithm>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#define G6_API extern "C" __attribute__((visibility("default")))
+#define G6_WEAK extern "C" __attribute__((visibility("default"), weak))
+
+namespace {
+
+using g6k::ForceCfg;
+using g6k::IBlock;
+using g6k::JRecord;
+using g6k::JStore;
+
+constexpr int kMaxPipes = 1 << 22;
+constexpr int kMaxSlots = 1 << 27;
+constexpr long long kPartBudget = 1ll << 21;  // nchunk * ni limit for partials
+
+struct TuneEntry {
+    int ni_lo, ni_hi;  // bucket [lo, hi]
+    int njs, ipl, nchunk;
+};
+
+struct Unit {
+    bool open = false;
+    bool have_device = false;
+    int device = -1;
+    hipStream_t stream = nullptr;
+
+    JStore js;
+    int nj_hw = 0;  // one past the highest slot ever written since reset
+    IBlock ib;
+
+    std::vector<JRecord> stage;
+    std::vector<int> stage_pos;  // slot -> index in stage, -1 if none
+    JRecord *h_stage = nullptr;
+    size_t h_stage_cap = 0;
+    JRecord *d_stage = nullptr;
+    size_t d_stage_cap = 0;
+    hipEvent_t ev_stage = nullptr;
+
+    double *h_in = nullptr;  // pinned [7][cap] x y z vx vy vz h2
+    int *h_id = nullptr;     // pinned [cap]
+    double *h_out = nullptr; // pinned [7][cap]
+    int *h_nn = nullptr;     // pinned [cap]
+    int h_cap = 0;
+    hipEvent_t ev_in = nullptr;
+
+    double ti = 0.0;
+    bool pred_valid = false;
+    double pred_ti = 0.0;
+    int pred_prec = -1;
+    int pred_compat = -1;
+    bool k18_nonzero = false;
+
+    int prec = g6k::kF64;
+    int compat = 0;
+    int npipes = 16384;
+    bool eager = false;
+
+    bool latched = false;
+    int first_rc = 0;  // error of the last firsthalf (it returns void)
+    int l_ni = 0, l_nj = 0;
+    double l_eps2 = 0.0;
+    std::vector<int> l_id;
+    bool launched = false;
+    bool launched_nn = false;
+    int last_ni = 0;  // rows valid in ib.out
+
+    int tile_njs = 0, tile_ipl = 0, tile_nchunk = 0;
+    std::vector<TuneEntry> tune;
+
+    bool timing = false;
+    struct Ev {
+        hipEvent_t a, b;
+        int kind;  // 0 force, 1 reduce, 2 predict
+        double pairs;
+    };
+    std::vector<Ev> evs;
+    std::vector<hipEvent_t> ev_pool;
+    double t_force = 0, t_reduce = 0, t_pred = 0, t_pairs = 0;
+    long long n_force = 0;
+
+    std::string err;
+};
+
+Unit U;
+
+int fail(int code, const char *what, hipError_t e = hipSuccess)
+{
+    char buf[256];
+    if (e != hipSuccess)
+        snprintf(buf, sizeof buf, "%s: %s", what, hipGetErrorString(e));
+    else
+        snprintf(buf, sizeof buf, "%s", what);
+    U.err = buf;
+    return code;
+}
+
+#define HIPCHK(expr, what)                                        \
+    do {                                                          \
+        hipError_t e_ = (expr);                                   \
+        if (e_ != hipSuccess) return fail(G6X_ERR_HIP, what, e_); \
+    } while (0)
+
+int env_int(const char *name, int dflt)
+{
+    const char *s = getenv(name);
+    if (!s || !*s) return dflt;
+    return atoi(s);
+}
+
+hipEvent_t pool_event()
+{
+    if (!U.ev_pool.empty()) {
+        hipEvent_t e = U.ev_pool.back();
+        U.ev_pool.pop_back();
+        return e;
+    }
+    hipEvent_t e = nullptr;
+    if (hipEventCreate(&e) != hipSuccess) return nullptr;
+    return e;
+}
+
+void timed_begin(Unit::Ev &ev, int kind, double pairs)
+{
+    ev.a = ev.b = nullptr;
+    ev.kind = kind;
+    ev.pairs = pairs;
+    if (!U.timing) return;
+    ev.a = pool_event();
+    ev.b = pool_event();
+    if (ev.a) hipEventRecord(ev.a, U.stream);
+}
+
+void timed_end(Unit::Ev &ev)
+{
+    if (!U.timing || !ev.a || !ev.b) return;
+    hipEventRecord(ev.b, U.stream);
+    U.evs.push_back(ev);
+}
+
+void harvest_timing()
+{
+    for (auto &ev : U.evs) {
+        hipEventSynchronize(ev.b);
+        float ms = 0.0f;
+        hipEventElapsedTime(&ms, ev.a, ev.b);
+        if (ev.kind == 0) {
+            U.t_force += ms;
+            U.t_pairs += ev.pairs;
+            U.n_force++;
+        } else if (ev.kind == 1) {
+            U.t_reduce += ms;
+        } else {
+            U.t_pred += ms;
+        }
+        U.ev_pool.push_back(ev.a);
+        U.ev_pool.push_back(ev.b);
+    }
+    U.evs.clear();
+}
+
+// ---------------------------------------------------------------------------
+// Device memory
+// ---------------------------------------------------------------------------
+int ensure_device()
+{
+    if (U.have_device) return 0;
+    int n = 0;
+    if (hipGetDeviceCount(&n) != hipSuccess || n <= 0) return fail(G6X_ERR_NODEVICE, "no GPU visible");
+    U.have_device = true;
+    return 0;
+}
+
+int grow_jstore(int need)
+{
+    if (need <= U.js.cap) return 0;
+    if (need > kMaxSlots) return fail(G6X_ERR_ARG, "j-slot address beyond 2^27");
+    int cap = std::max(U.js.cap ? U.js.cap : 1, 1 << 16);
+    while (cap < need) cap *= 2;
+    void *base = nullptr;
+    hipError_t e = hipMalloc(&base, g6k::jstore_bytes(cap));
+    if (e != hipSuccess) return fail(G6X_ERR_NOMEM, "hipMalloc j-store", e);
+    JStore ns;
+    g6k::jstore_carve(ns, base, cap);
+    const int old = U.js.cap;
+    if (old > 0) {
+        HIPCHK(g6k::launch_copy_store(ns, U.js, old, U.stream), "copy j-store");
+        HIPCHK(g6k::launch_reset(ns, old, cap, U.stream), "reset j-store");
+        HIPCHK(hipStreamSynchronize(U.stream), "grow sync");
+        hipFree(U.js.base);
+    } else {
+        HIPCHK(g6k::launch_reset(ns, 0, cap, U.stream), "reset j-store");
+    }
+    U.js = ns;
+    U.stage_pos.resize(cap, -1);
+    return 0;
+}
+
+int grow_iblock(int need)
+{
+    if (need <= U.ib.cap && need <= U.h_cap) return 0;
+    if (need > kMaxPipes) return fail(G6X_ERR_ARG, "i-block larger than 2^22");
+    int cap = std::max(U.ib.cap, 1024);
+    while (cap < need) cap *= 2;
+    HIPCHK(hipStreamSynchronize(U.stream), "iblock sync");
+    void *base = nullptr;
+    hipError_t e = hipMalloc(&base, g6k::iblock_bytes(cap));
+    if (e != hipSuccess) return fail(G6X_ERR_NOMEM, "hipMalloc i-block", e);
+    if (U.ib.base) hipFree(U.ib.base);
+    if (U.ib.part_base) hipFree(U.ib.part_base);
+    IBlock nb;
+    g6k::iblock_carve(nb, base, cap);
+    U.ib = nb;
+    if (U.h_in) { hipHostFree(U.h_in); hipHostFree(U.h_id); hipHostFree(U.h_out); hipHostFree(U.h_nn); }
+    U.h_in = nullptr; U.h_id = nullptr; U.h_out = nullptr; U.h_nn = nullptr;
+    e = hipHostMalloc(reinterpret_cast<void **>(&U.h_in), sizeof(double) * 7 * size_t(cap));
+    if (e == hipSuccess) e = hipHostMalloc(reinterpret_cast<void **>(&U.h_id), sizeof(int) * size_t(cap));
+    if (e == hipSuccess) e = hipHostMalloc(reinterpret_cast<void **>(&U.h_out), sizeof(double) * 7 * size_t(cap));
+    if (e == hipSuccess) e = hipHostMalloc(reinterpret_cast<void **>(&U.h_nn), sizeof(int) * size_t(cap));
+    if (e != hipSuccess) return fail(G6X_ERR_NOMEM, "hipHostMalloc i/o", e);
+    U.h_cap = cap;
+    return 0;
+}
+
+int ensure_partials(int nchunk)
+{
+    if (nchunk <= 1 || nchunk <= U.ib.part_chunks) return 0;
+    HIPCHK(hipStreamSynchronize(U.stream), "partials sync");
+    if (U.ib.part_base) hipFree(U.ib.part_base);
+    U.ib.part_base = nullptr;
+    U.ib.part = nullptr;
+    U.ib.part_chunks = 0;
+    void *p = nullptr;
+    hipError_t e = hipMalloc(&p, sizeof(double) * g6k::kPartFieldsD * size_t(U.ib.cap) * size_t(nchunk));
+    if (e != hipSuccess) return fail(G6X_ERR_NOMEM, "hipMalloc partials", e);
+    U.ib.part_base = p;
+    U.ib.part = static_cast<double *>(p);
+    U.ib.part_chunks = nchunk;
+    return 0;
+}
+
+// ---------------------------------------------------------------------------
+// j staging and prediction
+// ---------------------------------------------------------------------------
+int flush_stage()
+{
+    const size_t n = U.stage.size();
+    if (n == 0) return 0;
+    if (U.ev_stage) HIPCHK(hipEventSynchronize(U.ev_stage), "stage event");
+    if (n > U.h_stage_cap) {
+        if (U.h_stage) hipHostFree(U.h_stage);
+        size_t cap = std::max<size_t>(n, 4096);
+        hipError_t e = hipHostMalloc(reinterpret_cast<void **>(&U.h_stage), sizeof(JRecord) * cap);
+        if (e != hipSuccess) { U.h_stage = nullptr; U.h_stage_cap = 0; return fail(G6X_ERR_NOMEM, "hipHostMalloc stage", e); }
+        U.h_stage_cap = cap;
+    }
+    if (n > U.d_stage_cap) {
+        HIPCHK(hipStreamSynchronize(U.stream), "stage sync");
+        if (U.d_stage) hipFree(U.d_stage);
+        size_t cap = std::max<size_t>(n, 4096);
+        hipError_t e = hipMalloc(reinterpret_cast<void **>(&U.d_stage), sizeof(JRecord) * cap);
+        if (e != hipSuccess) { U.d_stage = nullptr; U.d_stage_cap = 0; return fail(G6X_ERR_NOMEM, "hipMalloc stage", e); }
+        U.d_stage_cap = cap;
+    }
+    memcpy(U.h_stage, U.stage.data(), sizeof(JRecord) * n);
+    HIPCHK(hipMemcpyAsync(U.d_stage, U.h_stage, sizeof(JRecord) * n, hipMemcpyHostToDevice, U.stream), "stage H2D");
+    const bool pv = U.pred_valid && U.pred_prec == U.prec && U.pred_compat == U.compat;
+    HIPCHK(g6k::launch_scatter(U.js, U.d_stage, int(n), U.prec, pv ? 1 : 0, U.pred_ti, U.compat, U.stream), "scatter");
+    if (!U.ev_stage) HIPCHK(hipEventCreateWithFlags(&U.ev_stage, hipEventDisableTiming), "event");
+    HIPCHK(hipEventRecord(U.ev_stage, U.stream), "stage record");
+    for (const auto &r : U.stage) U.stage_pos[r.addr] = -1;
+    U.stage.clear();
+    if (!pv) U.pred_valid = false;
+    return 0;
+}
+
+int ensure_predicted()
+{
+    if (U.pred_valid && U.pred_ti == U.ti && U.pred_prec == U.prec && U.pred_compat == U.compat) return 0;
+    Unit::Ev ev;
+    timed_begin(ev, 2, 0.0);
+    HIPCHK(g6k::launch_predict(U.js, U.nj_hw, U.ti, U.prec, U.compat, U.k18_nonzero ? 1 : 0, U.stream), "predict");
+    timed_end(ev);
+    U.pred_valid = true;
+    U.pred_ti = U.ti;
+    U.pred_prec = U.prec;
+    U.pred_compat = U.compat;
+    return 0;
+}
+
+// ---------------------------------------------------------------------------
+// Tile choice
+// ---------------------------------------------------------------------------
+ForceCfg choose_cfg(int ni, int nj)
+{
+    ForceCfg c;
+    int njs = 0, ipl = 0, nchunk = 0;
+    if (U.tile_njs > 0) {
+        njs = U.tile_njs; ipl = U.tile_ipl; nchunk = U.tile_nchunk;
+    } else {
+        for (const auto &t : U.tune)
+            if (ni >= t.ni_lo && ni <= t.ni_hi) { njs = t.njs; ipl = t.ipl; nchunk = t.nchunk; break; }
+    }
+    if (njs == 0) {
+        // Largest i-block that does not leave lanes idle, then enough
+        // j-chunks for ~8 workgroups per CU on 256 CUs.
+        static const int kOrder[][2] = {{1, 2}, {1, 1}, {2, 1}, {4, 1}, {8, 1}, {16, 1},
+                                        {32, 1}, {64, 1}, {128, 1}, {256, 1}};
+        int p2 = 1;
+        while (p2 < ni) p2 *= 2;
+        for (const auto &o : kOrder) {
+            const int ib = (g6k::kWG / o[0]) * o[1];
+            if (ib <= p2) { njs = o[0]; ipl = o[1]; break; }
+        }
+        if (njs == 0) { njs = 256; ipl = 1; }
+        nchunk = 0;
+    }
+    const int ib = (g6k::kWG / njs) * ipl;
+    const int iblocks = (ni + ib - 1) / ib;
+    const int max_chunks = std::max(1, (nj + g6k::kTJ - 1) / g6k::kTJ);
+    if (nchunk <= 0) nchunk = std::max(1, (2048 + iblocks - 1) / iblocks);
+    nchunk = std::min(nchunk, max_chunks);
+    while (nchunk > 1 && (long long)nchunk * ni > kPartBudget) nchunk /= 2;
+    int jchunk = (nj + nchunk - 1) / nchunk;
+    jchunk = ((jchunk + g6k::kTJ - 1) / g6k::kTJ) * g6k::kTJ;
+    if (jchunk <= 0) jchunk = g6k::kTJ;
+    nchunk = std::max(1, (nj + jchunk - 1) / jchunk);
+    c.njs = njs;
+    c.ipl = ipl;
+    c.nchunk = nchunk;
+    c.jchunk = jchunk;
+    return c;
+}
+
+int run_force(int ni, int nj, double eps2, bool want_nn)
+{
+    if (ni <= 0) { U.last_ni = 0; return 0; }
+    ForceCfg cfg = choose_cfg(ni, std::max(nj, 1));
+    int rc = ensure_partials(cfg.nchunk);
+    if (rc) return rc;
+    Unit::Ev ev;
+    timed_begin(ev, 0, double(ni) * double(nj));
+    HIPCHK(g6k::launch_force(U.js, U.ib, ni, nj, eps2, U.prec, want_nn, cfg, U.stream), "force kernel");
+    timed_end(ev);
+    if (cfg.nchunk > 1) {
+        Unit::Ev ev2;
+        timed_begin(ev2, 1, 0.0);
+        HIPCHK(g6k::launch_reduce(U.js, U.ib, ni, cfg.nchunk, U.prec, want_nn, U.stream), "reduce kernel");
+        timed_end(ev2);
+    }
+    U.launched = true;
+    U.launched_nn = want_nn;
+    U.last_ni = ni;
+    return 0;
+}
+
+int check_open()
+{
+    if (!U.open) return fail(G6X_ERR_NOT_OPEN, "g6 unit not open");
+    return 0;
+}
+
+int do_open(int cluster)
+{
+    int rc = ensure_device();
+    if (rc) return rc;
+    int ndev = 0;
+    HIPCHK(hipGetDeviceCount(&ndev), "device count");
+    int dev = env_int("G6_DEVICE", cluster);
+    if (dev < 0 || dev >= ndev) return fail(G6X_ERR_NODEVICE, "device id out of range");
+    if (U.stream && dev != U.device) {
+        // another device: drop the cached allocations of the old one
+        hipSetDevice(U.device);
+        hipStreamSynchronize(U.stream);
+        if (U.js.base) hipFree(U.js.base);
+        if (U.ib.base) hipFree(U.ib.base);
+        if (U.ib.part_base) hipFree(U.ib.part_base);
+        if (U.d_stage) hipFree(U.d_stage);
+        hipStreamDestroy(U.stream);
+        U.js = JStore();
+        U.ib = IBlock();
+        U.d_stage = nullptr;
+        U.d_stage_cap = 0;
+        U.stream = nullptr;
+        U.ev_stage = nullptr;
+        U.ev_pool.clear();
+        U.evs.clear();
+    }
+    HIPCHK(hipSetDevice(dev), "hipSetDevice");
+    U.device = dev;
+    if (!U.stream) HIPCHK(hipStreamCreateWithFlags(&U.stream, hipStreamNonBlocking), "stream");
+
+    const char *p = getenv("G6_PRECISION");
+    U.prec = (p && (!strcmp(p, "mixed") || !strcmp(p, "ds"))) ? g6k::kMixed : g6k::kF64;
+    const char *c = getenv("G6_PREDICTOR_COMPAT");
+    U.compat = (c && !strcmp(c, "g6lib")) ? 1 : 0;
+    U.npipes = std::min(std::max(env_int("G6_NPIPES", 16384), 1), kMaxPipes);
+    U.eager = env_int("G6_EAGER", 0) != 0;
+
+    rc = grow_jstore(std::max(env_int("G6_NJ_MAX", 1 << 16), 1));
+    if (rc) return rc;
+    rc = grow_iblock(std::min(U.npipes, 1 << 16));
+    if (rc) return rc;
+    HIPCHK(g6k::launch_reset(U.js, 0, U.js.cap, U.stream), "reset");
+    HIPCHK(hipStreamSynchronize(U.stream), "open sync");
+    U.stage.clear();
+    std::fill(U.stage_pos.begin(), U.stage_pos.end(), -1);
+    U.nj_hw = 0;
+    U.pred_valid = false;
+    U.k18_nonzero = false;
+    U.ti = 0.0;
+    U.latched = false;
+    U.launched = false;
+    U.open = true;
+    U.err.clear();
+    return 0;
+}
+
+int do_set_j(int address, int index, double tj, double dtj, double mass, const double *k18,
+             const double *j6, const double *a2, const double *v, const double *x)
+{
+    int rc = check_open();
+    if (rc) return rc;
+    if (address < 0) return fail(G6X_ERR_ARG, "negative j address");
+    if (address >= U.js.cap) {
+        rc = flush_stage();
+        if (rc) return rc;
+        rc = grow_jstore(address + 1);
+        if (rc) return rc;
+    }
+    JRecord r;
+    r.tj = tj; r.dtj = dtj; r.m = mass;
+    for (int k = 0; k < 3; k++) {
+        r.x[k] = x[k]; r.v[k] = v[k]; r.a2[k] = a2[k]; r.j6[k] = j6[k]; r.k18[k] = k18[k];
+        if (k18[k] != 0.0) U.k18_nonzero = true;
+    }
+    r.addr = address;
+    r.id = index;
+    int &pos = U.stage_pos[address];
+    if (pos >= 0) U.stage[pos] = r;
+    else { pos = int(U.stage.size()); U.stage.push_back(r); }
+    if (address + 1 > U.nj_hw) U.nj_hw = address + 1;
+    return 0;
+}
+
+int do_firsthalf(int nj, int ni, const int *index, const double (*xi)[3], const double (*vi)[3],
+                 double eps2, const double *h2)
+{
+    U.latched = false;
+    U.launched = false;
+    int rc = check_open();
+    if (rc) return rc;
+    if (ni < 0 || nj < 0) return fail(G6X_ERR_ARG, "negative ni or nj");
+    if (ni > kMaxPipes) return fail(G6X_ERR_ARG, "ni beyond 2^22");
+    if (nj > U.js.cap) {
+        rc = flush_stage();
+        if (rc) return rc;
+        rc = grow_jstore(nj);
+        if (rc) return rc;
+    }
+    if (nj > U.nj_hw) U.nj_hw = nj;  // slots never written are empty (id -1)
+    rc = grow_iblock(std::max(ni, 1));
+    if (rc) return rc;
+    rc = flush_stage();
+    if (rc) return rc;
+    rc = ensure_predicted();
+    if (rc) return rc;
+    if (ni > 0) {
+        if (U.ev_in) HIPCHK(hipEventSynchronize(U.ev_in), "i event");
+        const size_t C = size_t(U.ib.cap);
+        for (int i = 0; i < ni; i++) {
+            for (int k = 0; k < 3; k++) {
+                U.h_in[k * C + i] = xi[i][k];
+                U.h_in[(3 + k) * C + i] = vi[i][k];
+            }
+            U.h_in[6 * C + i] = h2 ? h2[i] : 0.0;
+            U.h_id[i] = index[i];
+        }
+        for (int k = 0; k < 3; k++) {
+            HIPCHK(hipMemcpyAsync(U.ib.x[k], U.h_in + k * C, sizeof(double) * ni, hipMemcpyHostToDevice, U.stream), "i H2D");
+            HIPCHK(hipMemcpyAsync(U.ib.v[k], U.h_in + (3 + k) * C, sizeof(double) * ni, hipMemcpyHostToDevice, U.stream), "i H2D");
+        }
+        HIPCHK(hipMemcpyAsync(U.ib.h2, U.h_in + 6 * C, sizeof(double) * ni, hipMemcpyHostToDevice, U.stream), "i H2D");
+        HIPCHK(hipMemcpyAsync(U.ib.id, U.h_id, sizeof(int) * ni, hipMemcpyHostToDevice, U.stream), "i H2D");
+        if (!U.ev_in) HIPCHK(hipEventCreateWithFlags(&U.ev_in, hipEventDisableTiming), "event");
+        HIPCHK(hipEventRecord(U.ev_in, U.stream), "i record");
+    }
+    U.l_ni = ni;
+    U.l_nj = nj;
+    U.l_eps2 = eps2;
+    U.l_id.assign(index, index + ni);
+    U.latched = true;
+    if (U.eager) {
+        rc = run_force(ni, nj, eps2, true);
+        if (rc) return rc;
+    }
+    return 0;
+}
+
+int fetch_results(int ni, double (*acc)[3], double (*jerk)[3], double *pot, int *nn, bool want_nn)
+{
+    if (ni <= 0) return 0;
+    const size_t C = size_t(U.ib.cap);
+    // ib.out rows are [7][C]; one strided copy brings all seven columns
+    HIPCHK(hipMemcpy2DAsync(U.h_out, sizeof(double) * C, U.ib.out, sizeof(double) * C,
+                            sizeof(double) * ni, 7, hipMemcpyDeviceToHost, U.stream), "out D2H");
+    if (want_nn) HIPCHK(hipMemcpyAsync(U.h_nn, U.ib.nn, sizeof(int) * ni, hipMemcpyDeviceToHost, U.stream), "nn D2H");
+    HIPCHK(hipStreamSynchronize(U.stream), "lasthalf sync");
+    for (int i = 0; i < ni; i++) {
+        for (int k = 0; k < 3; k++) {
+            acc[i][k] = U.h_out[k * C + i];
+            jerk[i][k] = U.h_out[(3 + k) * C + i];
+        }
+        pot[i] = U.h_out[6 * C + i];
+        if (want_nn) nn[i] = U.h_nn[i];
+    }
+    return 0;
+}
+
+int do_lasthalf(int *index, double (*acc)[3], double (*jerk)[3], double *pot, int *nn)
+{
+    int rc = check_open();
+    if (rc) return rc;
+    if (!U.latched) {
+        if (U.first_rc < 0) return U.first_rc;  // message of the failed firsthalf kept
+        return fail(G6X_ERR_SEQUENCE, "lasthalf without firsthalf");
+    }
+    const bool want_nn = nn != nullptr;
+    if (!U.launched || (want_nn && !U.launched_nn)) {
+        rc = run_force(U.l_ni, U.l_nj, U.l_eps2, want_nn);
+        if (rc) return rc;
+    }
+    rc = fetch_results(U.l_ni, acc, jerk, pot, nn, want_nn);
+    if (rc) return rc;
+    for (int i = 0; i < U.l_ni; i++) index[i] = U.l_id[i];  // g6lib.c:397,427
+    return 0;
+}
+
+}  // namespace
+
+// ===========================================================================
+// Fortran / by-pointer ABI
+// ===========================================================================
+G6_API int g6_open_(int *id) { return do_open(id ? *id : 0); }
+
+G6_API int g6_close_(int *)
+{
+    if (U.open && U.stream) hipStreamSynchronize(U.stream);
+    U.open = false;
+    U.latched = false;
+    return 0;
+}
+
+G6_API int g6_npipes_(void) { return U.npipes; }
+G6_API int g6_set_tunit_(void *) { return 0; }
+G6_API int g6_set_xunit_(void *) { return 0; }
+
+G6_API int g6_set_ti_(int *, double *ti)
+{
+    int rc = check_open();
+    if (rc) return rc;
+    U.ti = *ti;
+    return 0;
+}
+
+G6_API int g6_set_j_particle_(int *, int *address, int *index, double *tj, double *dtj,
+                              double *mass, double k18[3], double j6[3], double a2[3],
+                              double v[3], double x[3])
+{
+    return do_set_j(*address, *index, *tj, *dtj, *mass, k18, j6, a2, v, x);
+}
+
+G6_API void g6calc_firsthalf_(int *, int *nj, int *ni, int index[], double xi[][3],
+                              double vi[][3], double (*)[3], double (*)[3], double *,
+                              double *eps2, double h2[])
+{
+    U.first_rc = do_firsthalf(*nj, *ni, index, xi, vi, *eps2, h2);
+}
+
+G6_API int g6calc_lasthalf_(int *, int *, int *, int index[], double (*)[3], double (*)[3],
+                            double *, double *, double acc[][3], double jerk[][3], double pot[])
+{
+    return do_lasthalf(index, acc, jerk, pot, nullptr);
+}
+
+G6_API int g6calc_lasthalf2_(int *, int *, int *, int index[], double (*)[3], double (*)[3],
+                             double *, double *, double acc[][3], double jerk[][3],
+                             double pot[], int nnbindex[])
+{
+    return do_lasthalf(index, acc, jerk, pot, nnbindex);
+}
+
+G6_API int g6_initialize_jp_buffer_(int *, int *) { return 0; }
+G6_API int g6_flush_jp_buffer_(int *) { return 0; }
+
+G6_API int g6_reset_(int *)
+{
+    int rc = check_open();
+    if (rc) return rc;
+    U.stage.clear();
+    std::fill(U.stage_pos.begin(), U.stage_pos.end(), -1);
+    HIPCHK(g6k::launch_reset(U.js, 0, U.js.cap, U.stream), "reset");
+    HIPCHK(hipStreamSynchronize(U.stream), "reset sync");
+    U.nj_hw = 0;
+    U.pred_valid = false;
+    U.k18_nonzero = false;
+    return 0;
+}
+
+G6_API int g6_reset_fofpga_(int *) { return 0; }
+
+G6_API int g6_read_neighbour_list_(int *) { return 0; }
+
+G6_API int g6_get_neighbour_list_(int *, int *, int *, int *n_neighbours, int[])
+{
+    if (n_neighbours) *n_neighbours = 0;
+    return 0;
+}
+
+// ===========================================================================
+// C by-value ABI (weak: ph4's grape.cc defines the same wrappers)
+// ===========================================================================
+G6_WEAK int g6_open(int clusterid) { return g6_open_(&clusterid); }
+G6_WEAK int g6_close(int clusterid) { return g6_close_(&clusterid); }
+G6_WEAK int g6_npipes(void) { return g6_npipes_(); }
+G6_WEAK int g6_set_tunit(double) { return 0; }
+G6_WEAK int g6_set_xunit(double) { return 0; }
+G6_WEAK int g6_set_ti(int clusterid, double ti) { return g6_set_ti_(&clusterid, &ti); }
+G6_WEAK int g6_set_j_particle(int clusterid, int address, int index, double tj, double dtj,
+                              double mass, double a2by18[3], double a1by6[3], double aby2[3],
+                              double v[3], double x[3])
+{
+    return g6_set_j_particle_(&clusterid, &address, &index, &tj, &dtj, &mass, a2by18, a1by6, aby2,
+                              v, x);
+}
+G6_WEAK void g6calc_firsthalf(int clusterid, int nj, int ni, int index[], double xi[][3],
+                              double vi[][3], double fold[][3], double jold[][3], double phiold[],
+                              double eps2, double h2[])
+{
+    g6calc_firsthalf_(&clusterid, &nj, &ni, index, xi, vi, fold, jold, phiold, &eps2, h2);
+}
+G6_WEAK int g6calc_lasthalf(int clusterid, int nj, int ni, int index[], double xi[][3],
+                            double vi[][3], double eps2, double h2[], double acc[][3],
+                            double jerk[][3], double pot[])
+{
+    return g6calc_lasthalf_(&clusterid, &nj, &ni, index, xi, vi, &eps2, h2, acc, jerk, pot);
+}
+G6_WEAK int g6calc_lasthalf2(int clusterid, int nj, int ni, int index[], double xi[][3],
+                             double vi[][3], double eps2, double h2[], double acc[][3],
+                             double jerk[][3], double pot[], int nnbindex[])
+{
+    return g6calc_lasthalf2_(&clusterid, &nj, &ni, index, xi, vi, &eps2, h2, acc, jerk, pot,
+                             nnbindex);
+}
+G6_WEAK int g6_initialize_jp_buffer(int clusterid, int buf_size)
+{
+    return g6_initialize_jp_buffer_(&clusterid, &buf_size);
+}
+G6_WEAK int g6_flush_jp_buffer(int clusterid) { return g6_flush_jp_buffer_(&clusterid); }
+G6_WEAK int g6_reset(int clusterid) { return g6_reset_(&clusterid); }
+G6_WEAK int g6_reset_fofpga(int clusterid) { return g6_reset_fofpga_(&clusterid); }
+G6_WEAK int g6_read_neighbour_list(int clusterid) { return g6_read_neighbour_list_(&clusterid); }
+G6_WEAK int g6_get_neighbour_list(int clusterid, int ipipe, int maxlength, int *n_neighbours,
+                                  int neighbour_list[])
+{
+    return g6_get_neighbour_list_(&clusterid, &ipipe, &maxlength, n_neighbours, neighbour_list);
+}
+G6_WEAK void g6_set_neighbour_list_sort_mode(int) {}
+G6_WEAK int g6_get_neighbour_list_sort_mode(void) { return 1; }
+G6_WEAK int g6_set_overflow_flag_test_mode(int, int, int) { return 0; }
+G6_WEAK int g6_get_number_of_pipelines(void) { return g6_npipes_(); }
+G6_WEAK void g6_reinitialize(int clusterid) { g6_reset_(&clusterid); }
+
+G6_WEAK int get_device_count(void)
+{
+    int n = 0;
+    if (hipGetDeviceCount(&n) != hipSuccess) return 0;
+    return n;
+}
+G6_WEAK void force_j_particle_send(void) {}
+G6_WEAK int get_j_part_data(int, int, double *, double *, double *, double *, double *, double *)
+{
+    return 0;
+}
+
+// ===========================================================================
+// Extensions (g6x.h)
+// ===========================================================================
+G6_API int g6x_set_precision(int mode)
+{
+    if (mode != G6X_FP64 && mode != G6X_MIXED) return fail(G6X_ERR_ARG, "precision mode");
+    U.prec = mode;
+    return 0;
+}
+G6_API int g6x_get_precision(void) { return U.prec; }
+G6_API int g6x_set_predictor_compat(int on)
+{
+    U.compat = on ? 1 : 0;
+    return 0;
+}
+G6_API int g6x_set_npipes(int npipes)
+{
+    if (npipes < 1 || npipes > kMaxPipes) return fail(G6X_ERR_ARG, "npipes out of range");
+    U.npipes = npipes;
+    return 0;
+}
+G6_API const char *g6x_last_error(void) { return U.err.c_str(); }
+
+G6_API int g6x_calc_self(int nj, int i0, int ni, double eps2, int want_nn)
+{
+    int rc = check_open();
+    if (rc) return rc;
+    if (ni < 0 || i0 < 0 || nj < 0) return fail(G6X_ERR_ARG, "negative size");
+    if (nj > U.js.cap) {
+        rc = flush_stage();
+        if (rc) return rc;
+        rc = grow_jstore(nj);
+        if (rc) return rc;
+    }
+    if (nj > U.nj_hw) U.nj_hw = nj;
+    if ((long long)i0 + ni > U.js.cap) return fail(G6X_ERR_ARG, "i-range beyond j-store");
+    rc = grow_iblock(std::max(ni, 1));
+    if (rc) return rc;
+    rc = flush_stage();
+    if (rc) return rc;
+    rc = ensure_predicted();
+    if (rc) return rc;
+    HIPCHK(g6k::launch_iblock_self(U.js, U.ib, i0, ni, U.prec, U.stream), "iblock self");
+    U.latched = false;
+    return run_force(ni, nj, eps2, want_nn != 0);
+}
+
+G6_API int g6x_fetch(int ni, double acc[][3], double jerk[][3], double pot[], int nn[])
+{
+    int rc = check_open();
+    if (rc) return rc;
+    if (ni < 0 || ni > U.last_ni) return fail(G6X_ERR_ARG, "fetch beyond last result");
+    return fetch_results(ni, acc, jerk, pot, nn, nn != nullptr && U.launched_nn);
+}
+
+G6_API int g6x_sync(void)
+{
+    int rc = check_open();
+    if (rc) return rc;
+    HIPCHK(hipStreamSynchronize(U.stream), "sync");
+    return 0;
+}
+
+G6_API int g6x_timing(int enable)
+{
+    if (U.stream) hipStreamSynchronize(U.stream);
+    harvest_timing();
+    U.timing = enable != 0;
+    U.t_force = U.t_reduce = U.t_pred = U.t_pairs = 0.0;
+    U.n_force = 0;
+    return 0;
+}
+
+G6_API int g6x_get_timing(double *force_ms, double *reduce_ms, double *predict_ms,
+                          long long *force_launches, double *pairs)
+{
+    if (U.stream) hipStreamSynchronize(U.stream);
+    harvest_timing();
+    if (force_ms) *force_ms = U.t_force;
+    if (reduce_ms) *reduce_ms = U.t_reduce;
+    if (predict_ms) *predict_ms = U.t_pred;
+    if (force_launches) *force_launches = U.n_force;
+    if (pairs) *pairs = U.t_pairs;
+    return 0;
+}
+
+G6_API int g6x_set_tiles(int njs, int ipl, int nchunk)
+{
+    if (njs == 0) {
+        U.tile_njs = U.tile_ipl = U.tile_nchunk = 0;
+        return 0;
+    }
+    if (!g6k::force_cfg_supported(njs, ipl) || nchunk < 0) return fail(G6X_ERR_ARG, "tile config");
+    U.tile_njs = njs;
+    U.tile_ipl = ipl;
+    U.tile_nchunk = nchunk;
+    return 0;
+}
+
+G6_API int g6x_set_j_particles(int n, const int address[], const int index[], const double tj[],
+                               const double dtj[], const double mass[], const double k18[][3],
+                               const double j6[][3], const double a2[][3], const double v[][3],
+                               const double x[][3])
+{
+    if (n < 0) return fail(G6X_ERR_ARG, "negative count");
+    for (int q = 0; q < n; q++) {
+        int rc = do_set_j(address[q], index[q], tj[q], dtj[q], mass[q], k18[q], j6[q], a2[q], v[q],
+                          x[q]);
+        if (rc) return rc;
+    }
+    return 0;
+}
+
+// ---------------------------------------------------------------------------
+// Autotuner: for each power-of-two ni bucket up to ni_max, time every
+// compiled (njs, ipl) shape with the automatic chunk count and 2x / 0.5x
+// of it on the current j-store (i-block = the first ni predicted slots),
+// and keep the fastest.  Timing uses events on the unit's stream.
+// ---------------------------------------------------------------------------
+G6_API int g6x_autotune(int nj, int ni_max, double eps2)
+{
+    int rc = check_open();
+    if (rc) return rc;
+    if (nj <= 0 || ni_max <= 0 || ni_max > nj) return fail(G6X_ERR_ARG, "autotune sizes");
+    const int save_njs = U.tile_njs, save_ipl = U.tile_ipl, save_nc = U.tile_nchunk;
+    const bool save_timing = U.timing;
+    U.timing = false;
+    U.tune.clear();
+    static const int kShapes[][2] = {{1, 2}, {1, 1}, {2, 2}, {2, 1}, {4, 2}, {4, 1}, {8, 2},
+                                     {8, 1}, {16, 1}, {32, 1}, {64, 1}, {128, 1}, {256, 1}};
+    hipEvent_t a = nullptr, b = nullptr;
+    HIPCHK(hipEventCreate(&a), "event");
+    HIPCHK(hipEventCreate(&b), "event");
+    int timed = 0;
+    std::vector<TuneEntry> table;
+    for (int ni = 1; ni <= ni_max; ni *= 2) {
+        float best_ms = 1e30f;
+        TuneEntry best{ni, ni, 0, 0, 0};
+        for (const auto &sh : kShapes) {
+            const int ib = (g6k::kWG / sh[0]) * sh[1];
+            if (ib > 4 * ni && ib > 1) continue;  // mostly idle lanes
+            U.tile_njs = sh[0];
+            U.tile_ipl = sh[1];
+            U.tile_nchunk = 0;
+            const int auto_nc = choose_cfg(ni, nj).nchunk;
+            const int ncs[3] = {auto_nc, std::max(1, auto_nc / 2), auto_nc * 2};
+            for (int v = 0; v < 3; v++) {
+                if (v > 0 && ncs[v] == auto_nc) continue;
+                U.tile_nchunk = ncs[v];
+                rc = g6x_calc_self(nj, 0, ni, eps2, 1);  // warm-up / allocation
+                if (rc) goto done;
+                HIPCHK(hipEventRecord(a, U.stream), "record");
+                const int reps = 3;
+                for (int r = 0; r < reps; r++) {
+                    rc = run_force(ni, nj, eps2, true);
+                    if (rc) goto done;
+                }
+                HIPCHK(hipEventRecord(b, U.stream), "record");
+                HIPCHK(hipEventSynchronize(b), "sync");
+                float ms = 0.0f;
+                hipEventElapsedTime(&ms, a, b);
+                ms /= reps;
+                timed++;
+                if (ms < best_ms) {
+                    best_ms = ms;
+                    best.njs = sh[0];
+                    best.ipl = sh[1];
+                    best.nchunk = choose_cfg(ni, nj).nchunk;
+                }
+            }
+        }
+        table.push_back(best);
+    }
+    // buckets: (prev power, this power]
+    for (size_t k = 0; k < table.size(); k++) {
+        table[k].ni_lo = k == 0 ? 1 : table[k - 1].ni_hi + 1;
+    }
+    table.back().ni_hi = kMaxPipes;
+done:
+    hipEventDestroy(a);
+    hipEventDestroy(b);
+    U.tile_njs = save_njs;
+    U.tile_ipl = save_ipl;
+    U.tile_nchunk = save_nc;
+    U.timing = save_timing;
+    if (rc) return rc;
+    U.tune = table;
+    return timed;
+}
+
+G6_API int g6x_save_tuning(const char *path)
+{
+    FILE *f = fopen(path, "w");
+    if (!f) return fail(G6X_ERR_ARG, "cannot write tuning file");
+    fprintf(f, "# ni_lo ni_hi njs ipl nchunk\n");
+    for (const auto &t : U.tune) fprintf(f, "%d %d %d %d %d\n", t.ni_lo, t.ni_hi, t.njs, t.ipl, t.nchunk);
+    fclose(f);
+    return int(U.tune.size());
+}
+
+G6_API int g6x_load_tuning(const char *path)
+{
+    FILE *f = fopen(path, "r");
+    if (!f) return fail(G6X_ERR_ARG, "cannot read tuning file");
+    std::vector<TuneEntry> table;
+    char line[256];
+    while (fgets(line, sizeof line, f)) {
+        if (line[0] == '#') continue;
+        TuneEntry t;
+        if (sscanf(line, "%d %d %d %d %d", &t.ni_lo, &t.ni_hi, &t.njs, &t.ipl, &t.nchunk) == 5 &&
+            g6k::force_cfg_supported(t.njs, t.ipl) && t.ni_lo <= t.ni_hi && t.nchunk >= 0)
+            table.push_back(t);
+    }
+    fclose(f);
+    U.tune = table;
+    return int(table.size());
+}

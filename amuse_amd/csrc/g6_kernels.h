@@ -1,0 +1,95 @@
+// Device data layout and kernel launchers of the GRAPE-6 force path.
+//
+// HBM layout (all structure-of-arrays, one array per component, indexed by
+// j-slot so every kernel reads and writes contiguous, coalesced words):
+//
+//   base j-store (FP64, written by the scatter kernel from set_j_particle):
+//     x[3] v[3] a2[3] j6[3] k18[3] tj dtj m        17 doubles / slot
+//     id                                            int32
+//   predicted, plain path:   px[3] pv[3] pm         7 doubles / slot
+//   predicted, mixed path:   hx[3] lx[3] (double-single position, float hi
+//                            + float lo), fv[3], fm 10 floats / slot
+//   pid                       id, with empty slots (-1) mapped to zero mass
+//                            and a far position (shared by both paths)
+//   i-block (FP64):           x[3] v[3] h2, id, capacity = npipes
+//   results:                  out[7][cap] (acc xyz, jerk xyz, pot), nn[cap]
+//   partials (j-split over workgroups): part[chunk][16][cap] (plain path,
+//                            double-double) or [chunk][9][cap] (mixed)
+//
+// Reference for the arithmetic: lib/g6lib/g6lib.c:64-186 (predictor, pair
+// loop); the launch shapes are this library's own.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <cstdint>
+
+namespace g6k {
+
+constexpr int kWG = 256;        // threads per workgroup (4 wave64)
+constexpr int kTJ = 256;        // j per LDS tile
+constexpr int kPartFields = 9;   // mixed: (hi,lo) float pairs of acc3 jerk3 pot, r2, slot
+constexpr int kPartFieldsD = 16; // plain: hi, lo of acc3 jerk3 pot, r2, slot
+constexpr double kFar = 1.0e150;  // position given to empty slots
+
+struct JStore {
+    int cap = 0;
+    double *x[3] = {}, *v[3] = {}, *a2[3] = {}, *j6[3] = {}, *k18[3] = {};
+    double *tj = nullptr, *dtj = nullptr, *m = nullptr;
+    int *id = nullptr;
+    double *px[3] = {}, *pv[3] = {}, *pm = nullptr;
+    float *hx[3] = {}, *lx[3] = {}, *fv[3] = {}, *fm = nullptr;
+    int *pid = nullptr;
+    void *base = nullptr;   // single allocation holding everything above
+};
+
+// One staged set_j_particle call.
+struct JRecord {
+    double tj, dtj, m;
+    double x[3], v[3], a2[3], j6[3], k18[3];
+    int addr, id;
+};
+
+struct IBlock {
+    int cap = 0;
+    double *x[3] = {}, *v[3] = {}, *h2 = nullptr;
+    int *id = nullptr;
+    double *out = nullptr;   // [7][cap]
+    int *nn = nullptr;       // [cap]
+    double *part = nullptr;  // [nchunk_max][9][cap]
+    int part_chunks = 0;
+    void *base = nullptr;
+    void *part_base = nullptr;
+};
+
+enum Prec { kF64 = 0, kMixed = 1 };
+
+struct ForceCfg {
+    int njs = 1;     // lanes sharing one i-particle (j-split inside a WG)
+    int ipl = 1;     // i-particles per lane
+    int nchunk = 1;  // workgroups along j
+    int jchunk = 0;  // j per chunk (multiple of kTJ)
+};
+
+size_t jstore_bytes(int cap);
+void jstore_carve(JStore &s, void *base, int cap);
+size_t iblock_bytes(int cap);
+void iblock_carve(IBlock &b, void *base, int cap);
+
+hipError_t launch_reset(const JStore &s, int lo, int hi, hipStream_t st);
+hipError_t launch_copy_store(const JStore &dst, const JStore &src, int n,
+                             hipStream_t st);
+hipError_t launch_scatter(const JStore &s, const JRecord *rec, int n,
+                          int prec, int pred_valid, double ti, int compat,
+                          hipStream_t st);
+hipError_t launch_predict(const JStore &s, int nj, double ti, int prec,
+                          int compat, int use_k18, hipStream_t st);
+hipError_t launch_iblock_self(const JStore &s, const IBlock &b, int i0,
+                              int ni, int prec, hipStream_t st);
+// Valid (njs, ipl) pairs compiled in.
+bool force_cfg_supported(int njs, int ipl);
+hipError_t launch_force(const JStore &s, const IBlock &b, int ni, int nj,
+                        double eps2, int prec, bool want_nn,
+                        const ForceCfg &cfg, hipStream_t st);
+hipError_t launch_reduce(const JStore &s, const IBlock &b, int ni,
+                         int nchunk, int prec, bool want_nn, hipStream_t st);
+
+}  // namespace g6k

@@ -1,0 +1,905 @@
+// GRAPE-6 force path kernels for gfx950 (CDNA4, wave64).
+//
+//   reset / scatter / predict   j-store maintenance, HBM-streaming
+//   iblock_self                 gathers predicted j-slots as an i-block
+//   force_f64<NJS,IPL,NN>       plain path: FP64 differences, r^-1/2 and sums
+//   force_mx<NJS,IPL,NN>        mixed path: double-single dx, FP32 r^-3 and
+//                               sub-tile sums, Kahan/TwoSum outer accumulators
+//   reduce                      ordered sum of per-chunk partials + nn argmin
+//
+// Arithmetic follows lib/g6lib/g6lib.c: predictor :64-92, pair loop
+// :123-186 (skip id_i == id_j and id_j == -1, nearest neighbour on the
+// unsoftened r^2 with strict '<' so the lowest slot wins a tie, potential
+// -m/sqrt(r^2+eps2), acc m r / s^1.5, jerk m (v/s^1.5 - 3 (r.v) r / s^2.5)).
+//
+// Force kernel shape: a workgroup of 256 lanes covers IB = IPL*256/NJS
+// i-particles; NJS lanes share one i and take every NJS-th j of a 256-j tile
+// staged in LDS as structure-of-arrays (conflict-free ds_read_b128, a
+// broadcast when NJS = 1).  The NJS partial sums of one i are combined with
+// cross-lane shuffles (DPP / ds_swizzle), across waves through LDS when
+// NJS > 64.  Workgroups along grid.y split j into chunks; their partials are
+// summed in chunk order by the reduce kernel, so results do not depend on
+// scheduling.  The nearest-neighbour key is the pair (r^2, slot) compared
+// lexicographically, which reproduces the sequential first-wins rule for any
+// j-split.
+#include "g6_kernels.h"
+
+#include <cfloat>
+
+namespace g6k {
+
+static constexpr float kFarF = 1.0e18f;  // empty-slot position, mixed path
+
+// ------------------------------------------------------------------------
+// Memory carving
+// ------------------------------------------------------------------------
+static inline size_t align256(size_t b) { return (b + 255) & ~size_t(255); }
+
+size_t jstore_bytes(int cap)
+{
+    size_t n = size_t(cap);
+    size_t b = 0;
+    b += align256(n * sizeof(double)) * (15 + 3 + 7);  // base 15+tj,dtj,m ; pred 7
+    b += align256(n * sizeof(float)) * 10;             // mixed pred
+    b += align256(n * sizeof(int)) * 2;                // id, pid
+    return b;
+}
+
+void jstore_carve(JStore &s, void *base, int cap)
+{
+    char *p = static_cast<char *>(base);
+    size_t n = size_t(cap);
+    auto takeD = [&](double *&d) { d = reinterpret_cast<double *>(p); p += align256(n * sizeof(double)); };
+    auto takeF = [&](float *&d) { d = reinterpret_cast<float *>(p); p += align256(n * sizeof(float)); };
+    auto takeI = [&](int *&d) { d = reinterpret_cast<int *>(p); p += align256(n * sizeof(int)); };
+    for (int k = 0; k < 3; k++) { takeD(s.x[k]); takeD(s.v[k]); takeD(s.a2[k]); takeD(s.j6[k]); takeD(s.k18[k]); }
+    takeD(s.tj); takeD(s.dtj); takeD(s.m);
+    for (int k = 0; k < 3; k++) { takeD(s.px[k]); takeD(s.pv[k]); }
+    takeD(s.pm);
+    for (int k = 0; k < 3; k++) { takeF(s.hx[k]); takeF(s.lx[k]); takeF(s.fv[k]); }
+    takeF(s.fm);
+    takeI(s.id); takeI(s.pid);
+    s.cap = cap;
+    s.base = base;
+}
+
+size_t iblock_bytes(int cap)
+{
+    size_t n = size_t(cap);
+    return align256(n * sizeof(double)) * (7 + 7) + align256(n * sizeof(int)) * 2;
+}
+
+void iblock_carve(IBlock &b, void *base, int cap)
+{
+    char *p = static_cast<char *>(base);
+    size_t n = size_t(cap);
+    for (int k = 0; k < 3; k++) { b.x[k] = reinterpret_cast<double *>(p); p += align256(n * sizeof(double)); }
+    for (int k = 0; k < 3; k++) { b.v[k] = reinterpret_cast<double *>(p); p += align256(n * sizeof(double)); }
+    b.h2 = reinterpret_cast<double *>(p); p += align256(n * sizeof(double));
+    b.out = reinterpret_cast<double *>(p); p += align256(n * sizeof(double)) * 7;
+    b.id = reinterpret_cast<int *>(p); p += align256(n * sizeof(int));
+    b.nn = reinterpret_cast<int *>(p); p += align256(n * sizeof(int));
+    b.cap = cap;
+    b.base = base;
+}
+
+// ------------------------------------------------------------------------
+// Predictor (lib/g6lib/g6lib.c:64-92).  Contraction is off so each add and
+// multiply rounds once, as in the reference; the nearest-neighbour search
+// depends on predicted positions being bit-identical to it.
+// ------------------------------------------------------------------------
+struct Pred {
+    double x[3], v[3];
+};
+
+__device__ __forceinline__ Pred predict_slot(double ti, double tj,
+                                             const double x[3], const double v[3],
+                                             const double a2[3], const double j6[3],
+                                             const double k18[3], int compat, bool use_k18)
+{
+#pragma clang fp contract(off)
+    Pred r;
+    const double dt = ti - tj;
+    const double dt2 = dt * dt;
+    const double dt3 = dt2 * dt;
+    const double dt4 = dt2 * dt2;
+#pragma unroll
+    for (int k = 0; k < 3; k++) {
+        double dx = v[k] * dt;
+        dx = dx + a2[k] * dt2;
+        dx = dx + j6[k] * dt3;
+        if (use_k18) dx = dx + k18[k] * 3.0 / 4.0 * dt4;
+        r.x[k] = dx + x[k];
+        double dv = a2[k] * 2.0 * dt;
+        if (compat) {
+            dv = dv + j6[k] * 3.0 * dt3;
+            if (use_k18) dv = dv + k18[k] * 6.0 * dt4;
+        } else {
+            dv = dv + j6[k] * 3.0 * dt2;
+            if (use_k18) dv = dv + k18[k] * 3.0 * dt3;
+        }
+        r.v[k] = dv + v[k];
+    }
+    return r;
+}
+
+__device__ __forceinline__ void write_pred(const JStore &s, int j, int prec, int id, const Pred &p)
+{
+    if (prec == kF64) {
+#pragma unroll
+        for (int k = 0; k < 3; k++) {
+            s.px[k][j] = id == -1 ? kFar : p.x[k];
+            s.pv[k][j] = id == -1 ? 0.0 : p.v[k];
+        }
+    } else {
+#pragma unroll
+        for (int k = 0; k < 3; k++) {
+            const double xp = p.x[k];
+            const float hi = float(xp);
+            const float lo = float(xp - double(hi));
+            s.hx[k][j] = id == -1 ? kFarF : hi;
+            s.lx[k][j] = id == -1 ? 0.0f : lo;
+            s.fv[k][j] = id == -1 ? 0.0f : float(p.v[k]);
+        }
+    }
+}
+
+__global__ void __launch_bounds__(256) k_reset(JStore s, int lo, int hi)
+{
+    const int j = lo + blockIdx.x * blockDim.x + threadIdx.x;
+    if (j >= hi) return;
+#pragma unroll
+    for (int k = 0; k < 3; k++) {
+        s.x[k][j] = 0.0; s.v[k][j] = 0.0; s.a2[k][j] = 0.0; s.j6[k][j] = 0.0; s.k18[k][j] = 0.0;
+        s.px[k][j] = kFar; s.pv[k][j] = 0.0;
+        s.hx[k][j] = kFarF; s.lx[k][j] = 0.0f; s.fv[k][j] = 0.0f;
+    }
+    s.tj[j] = 0.0; s.dtj[j] = 0.0; s.m[j] = 0.0; s.pm[j] = 0.0; s.fm[j] = 0.0f;
+    s.id[j] = -1; s.pid[j] = -1;
+}
+
+__global__ void __launch_bounds__(256) k_copy_store(JStore d, JStore s, int n)
+{
+    const int j = blockIdx.x * blockDim.x + threadIdx.x;
+    if (j >= n) return;
+#pragma unroll
+    for (int k = 0; k < 3; k++) {
+        d.x[k][j] = s.x[k][j]; d.v[k][j] = s.v[k][j]; d.a2[k][j] = s.a2[k][j];
+        d.j6[k][j] = s.j6[k][j]; d.k18[k][j] = s.k18[k][j];
+        d.px[k][j] = s.px[k][j]; d.pv[k][j] = s.pv[k][j];
+        d.hx[k][j] = s.hx[k][j]; d.lx[k][j] = s.lx[k][j]; d.fv[k][j] = s.fv[k][j];
+    }
+    d.tj[j] = s.tj[j]; d.dtj[j] = s.dtj[j]; d.m[j] = s.m[j]; d.pm[j] = s.pm[j]; d.fm[j] = s.fm[j];
+    d.id[j] = s.id[j]; d.pid[j] = s.pid[j];
+}
+
+// Applies staged set_j_particle records; when the predicted arrays are
+// current for time ti they are updated for the touched slots as well, so
+// a later calc at the same ti needs no full predictor pass.
+__global__ void __launch_bounds__(256) k_scatter(JStore s, const JRecord *__restrict__ rec, int n,
+                                                 int prec, int pred_valid, double ti, int compat)
+{
+    const int r = blockIdx.x * blockDim.x + threadIdx.x;
+    if (r >= n) return;
+    const JRecord q = rec[r];
+    const int j = q.addr;
+#pragma unroll
+    for (int k = 0; k < 3; k++) {
+        s.x[k][j] = q.x[k]; s.v[k][j] = q.v[k]; s.a2[k][j] = q.a2[k];
+        s.j6[k][j] = q.j6[k]; s.k18[k][j] = q.k18[k];
+    }
+    s.tj[j] = q.tj; s.dtj[j] = q.dtj; s.m[j] = q.m; s.id[j] = q.id;
+    const double mp = q.id == -1 ? 0.0 : q.m;
+    s.pm[j] = mp;
+    s.fm[j] = float(mp);
+    s.pid[j] = q.id;
+    if (pred_valid) {
+        const Pred p = predict_slot(ti, q.tj, q.x, q.v, q.a2, q.j6, q.k18, compat, true);
+        write_pred(s, j, prec, q.id, p);
+    }
+}
+
+// Full predictor pass over slots [0, nj): streams 13 doubles + 1 int in
+// and 6 doubles (plain) or 9 floats (mixed) out per slot; HBM-bound.
+__global__ void __launch_bounds__(256) k_predict(JStore s, int nj, double ti, int prec, int compat,
+                                                 int use_k18)
+{
+    const int j = blockIdx.x * blockDim.x + threadIdx.x;
+    if (j >= nj) return;
+    double x[3], v[3], a2[3], j6[3], k18[3] = {0.0, 0.0, 0.0};
+#pragma unroll
+    for (int k = 0; k < 3; k++) {
+        x[k] = s.x[k][j]; v[k] = s.v[k][j]; a2[k] = s.a2[k][j]; j6[k] = s.j6[k][j];
+    }
+    if (use_k18) {
+#pragma unroll
+        for (int k = 0; k < 3; k++) k18[k] = s.k18[k][j];
+    }
+    const int id = s.id[j];
+    const Pred p = predict_slot(ti, s.tj[j], x, v, a2, j6, k18, compat, use_k18 != 0);
+    write_pred(s, j, prec, id, p);
+}
+
+__global__ void __launch_bounds__(256) k_iblock_self(JStore s, IBlock b, int i0, int ni, int prec)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= ni) return;
+    const int j = i0 + i;
+#pragma unroll
+    for (int k = 0; k < 3; k++) {
+        if (prec == kF64) {
+            b.x[k][i] = s.px[k][j];
+            b.v[k][i] = s.pv[k][j];
+        } else {
+            b.x[k][i] = double(s.hx[k][j]) + double(s.lx[k][j]);
+            b.v[k][i] = double(s.fv[k][j]);
+        }
+    }
+    b.h2[i] = 0.0;
+    b.id[i] = s.id[j];
+}
+
+// ------------------------------------------------------------------------
+// Pair interaction, plain FP64.
+// ------------------------------------------------------------------------
+// r^-1/2 to full double precision: v_rsq_f64 seed, then one third-order
+// correction y(1 + e/2 + 3e^2/8) with e = 1 - s y^2.
+__device__ __forceinline__ double rsqrt_f64(double s)
+{
+    const double y = __builtin_amdgcn_rsq(s);
+    const double h = s * y;
+    const double e = __builtin_fma(-h, y, 1.0);
+    const double t = __builtin_fma(e, 0.375, 0.5);
+    return __builtin_fma(y * e, t, y);
+}
+
+// r^2 rounded exactly as the reference: ((dx*dx + dy*dy) + dz*dz).
+__device__ __forceinline__ double exact_r2(double dx, double dy, double dz)
+{
+#pragma clang fp contract(off)
+    double r2 = dx * dx;
+    r2 = r2 + dy * dy;
+    r2 = r2 + dz * dz;
+    return r2;
+}
+
+struct AccD {
+    double ax, ay, az, jx, jy, jz, pot, best;
+    int addr;
+};
+
+struct ID {
+    double x, y, z, vx, vy, vz;
+    int id;
+};
+
+// double-double accumulator (hi + lo) fed with TwoSum: the plain per-lane
+// sums run over at most a few hundred pairs before being folded in, so the
+// error of a full sum does not grow with nj.
+struct DDAcc {
+    double h[7], l[7];
+};
+
+__device__ __forceinline__ void two_sum_add_d(double &hi, double &lo, double x)
+{
+    const double s = hi + x;
+    const double bp = s - hi;
+    const double err = (hi - (s - bp)) + (x - bp);
+    hi = s;
+    lo += err;
+}
+
+__device__ __forceinline__ void flush_f64(AccD &a, DDAcc &d)
+{
+    two_sum_add_d(d.h[0], d.l[0], a.ax);
+    two_sum_add_d(d.h[1], d.l[1], a.ay);
+    two_sum_add_d(d.h[2], d.l[2], a.az);
+    two_sum_add_d(d.h[3], d.l[3], a.jx);
+    two_sum_add_d(d.h[4], d.l[4], a.jy);
+    two_sum_add_d(d.h[5], d.l[5], a.jz);
+    two_sum_add_d(d.h[6], d.l[6], a.pot);
+    a.ax = a.ay = a.az = a.jx = a.jy = a.jz = a.pot = 0.0;
+}
+
+template <bool NN>
+__device__ __forceinline__ void pair_f64(const ID &I, AccD &a, double xj, double yj, double zj,
+                                         double mj, double vxj, double vyj, double vzj, int idj,
+                                         int addr, double eps2)
+{
+    const double dx = xj - I.x, dy = yj - I.y, dz = zj - I.z;
+    double r2, s;
+    if (NN) {
+        r2 = exact_r2(dx, dy, dz);
+        s = r2 + eps2;
+    } else {
+        r2 = 0.0;
+        s = __builtin_fma(dz, dz, __builtin_fma(dy, dy, __builtin_fma(dx, dx, eps2)));
+    }
+    const bool valid = idj != I.id;
+    double y = rsqrt_f64(s);
+    y = valid ? y : 0.0;
+    if (NN) {
+        const bool better = valid && (r2 < a.best);
+        a.best = better ? r2 : a.best;
+        a.addr = better ? addr : a.addr;
+    }
+    const double y2 = y * y;
+    const double mi = mj * y;
+    const double mi3 = mi * y2;
+    a.pot -= mi;
+    a.ax = __builtin_fma(mi3, dx, a.ax);
+    a.ay = __builtin_fma(mi3, dy, a.ay);
+    a.az = __builtin_fma(mi3, dz, a.az);
+    const double dvx = vxj - I.vx, dvy = vyj - I.vy, dvz = vzj - I.vz;
+    const double rv = __builtin_fma(dz, dvz, __builtin_fma(dy, dvy, dx * dvx));
+    const double c = (-3.0 * rv) * y2;
+    a.jx = __builtin_fma(mi3, __builtin_fma(c, dx, dvx), a.jx);
+    a.jy = __builtin_fma(mi3, __builtin_fma(c, dy, dvy), a.jy);
+    a.jz = __builtin_fma(mi3, __builtin_fma(c, dz, dvz), a.jz);
+}
+
+// Lexicographic (r2, slot) minimum; an empty key has r2 = +inf, slot = -1
+// (as unsigned, the largest).
+__device__ __forceinline__ void nn_merge(double &best, int &addr, double ob, int oa)
+{
+    const bool take = (ob < best) || (ob == best && unsigned(oa) < unsigned(addr));
+    best = take ? ob : best;
+    addr = take ? oa : addr;
+}
+
+__device__ __forceinline__ void nn_merge_f(float &best, int &addr, float ob, int oa)
+{
+    const bool take = (ob < best) || (ob == best && unsigned(oa) < unsigned(addr));
+    best = take ? ob : best;
+    addr = take ? oa : addr;
+}
+
+struct FParams {
+    const double *ix, *iy, *iz, *ivx, *ivy, *ivz;
+    const int *iid;
+    int ni;
+    // plain-path j
+    const double *jx, *jy, *jz, *jm, *jvx, *jvy, *jvz;
+    // mixed-path j
+    const float *hx, *hy, *hz, *lx, *ly, *lz, *fm, *fvx, *fvy, *fvz;
+    const int *jid;
+    int nj, jchunk;
+    double eps2;
+    double *out;  // [7][ostride]
+    int *nn;
+    int ostride;
+    double *part;  // [chunk][9][ostride]
+};
+
+// Shared-memory layout of one j tile.
+struct TileD {
+    double2 xy[kTJ], zm[kTJ], vxy[kTJ], vzid[kTJ];
+};
+struct TileF {
+    float4 a[kTJ];  // hx hy hz m
+    float4 b[kTJ];  // lx ly lz vx
+    float4 c[kTJ];  // vy vz id -
+};
+
+template <int NJS, int IPL, bool NN>
+__global__ void __launch_bounds__(kWG) k_force_f64(FParams p)
+{
+    constexpr int IBL = kWG / NJS;
+    constexpr int IB = IBL * IPL;
+    // tiles between two folds of the plain sums: >= 128 pairs per lane
+    constexpr int FT = NJS >= 4 ? NJS / 2 : 1;
+    __shared__ TileD tile;
+
+    const int t = threadIdx.x;
+    const int js = t % NJS;
+    const int il = t / NJS;
+    const int ibase = blockIdx.x * IB;
+
+    ID I[IPL];
+    AccD A[IPL];
+    DDAcc D[IPL];
+#pragma unroll
+    for (int q = 0; q < IPL; q++) {
+        const int i = min(ibase + il + q * IBL, p.ni - 1);
+        I[q].x = p.ix[i]; I[q].y = p.iy[i]; I[q].z = p.iz[i];
+        I[q].vx = p.ivx[i]; I[q].vy = p.ivy[i]; I[q].vz = p.ivz[i];
+        I[q].id = p.iid[i];
+        A[q] = AccD{0, 0, 0, 0, 0, 0, 0, __builtin_inf(), -1};
+#pragma unroll
+        for (int k = 0; k < 7; k++) { D[q].h[k] = 0.0; D[q].l[k] = 0.0; }
+    }
+    const double eps2 = p.eps2;
+    const int j0 = blockIdx.y * p.jchunk;
+    const int j1 = min(p.nj, j0 + p.jchunk);
+
+    int tcount = 0;
+    for (int tb = j0; tb < j1; tb += kTJ) {
+        const int jn = min(kTJ, j1 - tb);
+        __syncthreads();
+        if (t < jn) {
+            const int j = tb + t;
+            tile.xy[t] = make_double2(p.jx[j], p.jy[j]);
+            tile.zm[t] = make_double2(p.jz[j], p.jm[j]);
+            tile.vxy[t] = make_double2(p.jvx[j], p.jvy[j]);
+            tile.vzid[t] = make_double2(p.jvz[j], __longlong_as_double((long long)p.jid[j]));
+        }
+        __syncthreads();
+#pragma unroll 2
+        for (int jj = js; jj < jn; jj += NJS) {
+            const double2 a = tile.xy[jj], b = tile.zm[jj], c = tile.vxy[jj], d = tile.vzid[jj];
+            const int idj = int(__double_as_longlong(d.y));
+#pragma unroll
+            for (int q = 0; q < IPL; q++)
+                pair_f64<NN>(I[q], A[q], a.x, a.y, b.x, b.y, c.x, c.y, d.x, idj, tb + jj, eps2);
+        }
+        if (++tcount == FT) {
+            tcount = 0;
+#pragma unroll
+            for (int q = 0; q < IPL; q++) flush_f64(A[q], D[q]);
+        }
+    }
+#pragma unroll
+    for (int q = 0; q < IPL; q++) flush_f64(A[q], D[q]);
+
+    // Combine the NJS lanes of each i (double-double adds).
+#pragma unroll
+    for (int q = 0; q < IPL; q++) {
+#pragma unroll
+        for (int off = 1; off < (NJS < 64 ? NJS : 64); off <<= 1) {
+#pragma unroll
+            for (int k = 0; k < 7; k++) {
+                const double oh = __shfl_xor(D[q].h[k], off);
+                const double ol = __shfl_xor(D[q].l[k], off);
+                two_sum_add_d(D[q].h[k], D[q].l[k], oh);
+                D[q].l[k] += ol;
+            }
+            if (NN) {
+                const double ob = __shfl_xor(A[q].best, off);
+                const int oa = __shfl_xor(A[q].addr, off);
+                nn_merge(A[q].best, A[q].addr, ob, oa);
+            }
+        }
+    }
+    if (NJS > 64) {
+        // Lanes 0 of the NJS/64 waves of each i meet in LDS.
+        constexpr int W = NJS > 64 ? NJS / 64 : 1;
+        __syncthreads();
+        double *red = reinterpret_cast<double *>(&tile);  // [wave][IPL][16]
+        const int wave = t / 64;
+        if ((t & 63) == 0) {
+#pragma unroll
+            for (int q = 0; q < IPL; q++) {
+                double *r = red + (wave * IPL + q) * 16;
+#pragma unroll
+                for (int k = 0; k < 7; k++) { r[k] = D[q].h[k]; r[7 + k] = D[q].l[k]; }
+                r[14] = A[q].best;
+                r[15] = double(A[q].addr);
+            }
+        }
+        __syncthreads();
+        if (js == 0) {
+#pragma unroll
+            for (int q = 0; q < IPL; q++) {
+                for (int w = 1; w < W; w++) {
+                    const double *r = red + ((wave + w) * IPL + q) * 16;
+#pragma unroll
+                    for (int k = 0; k < 7; k++) {
+                        two_sum_add_d(D[q].h[k], D[q].l[k], r[k]);
+                        D[q].l[k] += r[7 + k];
+                    }
+                    if (NN) nn_merge(A[q].best, A[q].addr, r[14], int(r[15]));
+                }
+            }
+        }
+    }
+    if (js != 0) return;
+#pragma unroll
+    for (int q = 0; q < IPL; q++) {
+        const int i = ibase + il + q * IBL;
+        if (i >= p.ni) continue;
+        const int S = p.ostride;
+        if (gridDim.y == 1) {
+#pragma unroll
+            for (int k = 0; k < 7; k++) p.out[k * S + i] = D[q].h[k] + D[q].l[k];
+            if (NN) p.nn[i] = A[q].addr >= 0 ? p.jid[A[q].addr] : -1;
+        } else {
+            // partial record: hi and lo of each sum in consecutive field pairs
+            double *r = p.part + size_t(blockIdx.y) * kPartFieldsD * S;
+#pragma unroll
+            for (int k = 0; k < 7; k++) {
+                r[(2 * k) * S + i] = D[q].h[k];
+                r[(2 * k + 1) * S + i] = D[q].l[k];
+            }
+            r[14 * S + i] = NN ? A[q].best : __builtin_inf();
+            r[15 * S + i] = NN ? double(A[q].addr) : -1.0;
+        }
+    }
+}
+
+// ------------------------------------------------------------------------
+// Pair interaction, mixed precision.
+// ------------------------------------------------------------------------
+struct IF {
+    float hx, hy, hz, lx, ly, lz, vx, vy, vz;
+    int id;
+};
+struct AccF {
+    // sub-tile FP32 sums
+    float ax, ay, az, jx, jy, jz, pot;
+    float best;
+    int addr;
+};
+// double-single accumulator (hi + lo), fed with TwoSum
+struct DSAcc {
+    float h[7], l[7];
+};
+
+__device__ __forceinline__ void two_sum_add(float &hi, float &lo, float x)
+{
+    const float s = hi + x;
+    const float bp = s - hi;
+    const float err = (hi - (s - bp)) + (x - bp);
+    hi = s;
+    lo += err;
+}
+
+template <bool NN>
+__device__ __forceinline__ void pair_mx(const IF &I, AccF &a, float4 ja, float4 jb, float4 jc,
+                                        int addr, float eps2)
+{
+    const float dx = (ja.x - I.hx) + (jb.x - I.lx);
+    const float dy = (ja.y - I.hy) + (jb.y - I.ly);
+    const float dz = (ja.z - I.hz) + (jb.z - I.lz);
+    const int idj = __float_as_int(jc.z);
+    float r2, s;
+    if (NN) {
+        r2 = __builtin_fmaf(dz, dz, __builtin_fmaf(dy, dy, dx * dx));
+        s = r2 + eps2;
+    } else {
+        r2 = 0.0f;
+        s = __builtin_fmaf(dz, dz, __builtin_fmaf(dy, dy, __builtin_fmaf(dx, dx, eps2)));
+    }
+    const bool valid = idj != I.id;
+    float y = __builtin_amdgcn_rsqf(s);
+    y = valid ? y : 0.0f;
+    if (NN) {
+        const bool better = valid && (r2 < a.best);
+        a.best = better ? r2 : a.best;
+        a.addr = better ? addr : a.addr;
+    }
+    const float y2 = y * y;
+    const float mi = ja.w * y;
+    const float mi3 = mi * y2;
+    a.pot -= mi;
+    a.ax = __builtin_fmaf(mi3, dx, a.ax);
+    a.ay = __builtin_fmaf(mi3, dy, a.ay);
+    a.az = __builtin_fmaf(mi3, dz, a.az);
+    const float dvx = jb.w - I.vx, dvy = jc.x - I.vy, dvz = jc.y - I.vz;
+    const float rv = __builtin_fmaf(dz, dvz, __builtin_fmaf(dy, dvy, dx * dvx));
+    const float c = (-3.0f * rv) * y2;
+    a.jx = __builtin_fmaf(mi3, __builtin_fmaf(c, dx, dvx), a.jx);
+    a.jy = __builtin_fmaf(mi3, __builtin_fmaf(c, dy, dvy), a.jy);
+    a.jz = __builtin_fmaf(mi3, __builtin_fmaf(c, dz, dvz), a.jz);
+}
+
+__device__ __forceinline__ void flush_mx(AccF &a, DSAcc &d)
+{
+    two_sum_add(d.h[0], d.l[0], a.ax);
+    two_sum_add(d.h[1], d.l[1], a.ay);
+    two_sum_add(d.h[2], d.l[2], a.az);
+    two_sum_add(d.h[3], d.l[3], a.jx);
+    two_sum_add(d.h[4], d.l[4], a.jy);
+    two_sum_add(d.h[5], d.l[5], a.jz);
+    two_sum_add(d.h[6], d.l[6], a.pot);
+    a.ax = a.ay = a.az = a.jx = a.jy = a.jz = a.pot = 0.0f;
+}
+
+// pairs per lane between two flushes of the FP32 sub-tile sums
+constexpr int kFlush = 64;
+
+template <int NJS, int IPL, bool NN>
+__global__ void __launch_bounds__(kWG) k_force_mx(FParams p)
+{
+    constexpr int IBL = kWG / NJS;
+    constexpr int IB = IBL * IPL;
+    __shared__ TileF tile;
+
+    const int t = threadIdx.x;
+    const int js = t % NJS;
+    const int il = t / NJS;
+    const int ibase = blockIdx.x * IB;
+
+    IF I[IPL];
+    AccF A[IPL];
+    DSAcc D[IPL];
+#pragma unroll
+    for (int q = 0; q < IPL; q++) {
+        const int i = min(ibase + il + q * IBL, p.ni - 1);
+        const double x = p.ix[i], y = p.iy[i], z = p.iz[i];
+        I[q].hx = float(x); I[q].lx = float(x - double(I[q].hx));
+        I[q].hy = float(y); I[q].ly = float(y - double(I[q].hy));
+        I[q].hz = float(z); I[q].lz = float(z - double(I[q].hz));
+        I[q].vx = float(p.ivx[i]); I[q].vy = float(p.ivy[i]); I[q].vz = float(p.ivz[i]);
+        I[q].id = p.iid[i];
+        A[q] = AccF{0, 0, 0, 0, 0, 0, 0, __builtin_inff(), -1};
+#pragma unroll
+        for (int k = 0; k < 7; k++) { D[q].h[k] = 0.0f; D[q].l[k] = 0.0f; }
+    }
+    const float eps2 = float(p.eps2);
+    const int j0 = blockIdx.y * p.jchunk;
+    const int j1 = min(p.nj, j0 + p.jchunk);
+
+    for (int tb = j0; tb < j1; tb += kTJ) {
+        const int jn = min(kTJ, j1 - tb);
+        __syncthreads();
+        if (t < jn) {
+            const int j = tb + t;
+            tile.a[t] = make_float4(p.hx[j], p.hy[j], p.hz[j], p.fm[j]);
+            tile.b[t] = make_float4(p.lx[j], p.ly[j], p.lz[j], p.fvx[j]);
+            tile.c[t] = make_float4(p.fvy[j], p.fvz[j], __int_as_float(p.jid[j]), 0.0f);
+        }
+        __syncthreads();
+        for (int jb = js; jb < jn; jb += NJS * kFlush) {
+            const int je = min(jn, jb + NJS * kFlush);
+#pragma unroll 2
+            for (int jj = jb; jj < je; jj += NJS) {
+                const float4 a = tile.a[jj], b = tile.b[jj], c = tile.c[jj];
+#pragma unroll
+                for (int q = 0; q < IPL; q++) pair_mx<NN>(I[q], A[q], a, b, c, tb + jj, eps2);
+            }
+#pragma unroll
+            for (int q = 0; q < IPL; q++) flush_mx(A[q], D[q]);
+        }
+    }
+
+    // Combine the NJS lanes of each i (double-single adds).
+#pragma unroll
+    for (int q = 0; q < IPL; q++) {
+#pragma unroll
+        for (int off = 1; off < (NJS < 64 ? NJS : 64); off <<= 1) {
+#pragma unroll
+            for (int k = 0; k < 7; k++) {
+                const float oh = __shfl_xor(D[q].h[k], off);
+                const float ol = __shfl_xor(D[q].l[k], off);
+                two_sum_add(D[q].h[k], D[q].l[k], oh);
+                D[q].l[k] += ol;
+            }
+            if (NN) {
+                const float ob = __shfl_xor(A[q].best, off);
+                const int oa = __shfl_xor(A[q].addr, off);
+                nn_merge_f(A[q].best, A[q].addr, ob, oa);
+            }
+        }
+    }
+    if (NJS > 64) {
+        constexpr int W = NJS > 64 ? NJS / 64 : 1;
+        __syncthreads();
+        float *red = reinterpret_cast<float *>(&tile);  // [wave][IPL][16]
+        const int wave = t / 64;
+        if ((t & 63) == 0) {
+#pragma unroll
+            for (int q = 0; q < IPL; q++) {
+                float *r = red + (wave * IPL + q) * 16;
+#pragma unroll
+                for (int k = 0; k < 7; k++) { r[k] = D[q].h[k]; r[7 + k] = D[q].l[k]; }
+                r[14] = A[q].best;
+                r[15] = __int_as_float(A[q].addr);
+            }
+        }
+        __syncthreads();
+        if (js == 0) {
+#pragma unroll
+            for (int q = 0; q < IPL; q++) {
+                for (int w = 1; w < W; w++) {
+                    const float *r = red + ((wave + w) * IPL + q) * 16;
+#pragma unroll
+                    for (int k = 0; k < 7; k++) {
+                        two_sum_add(D[q].h[k], D[q].l[k], r[k]);
+                        D[q].l[k] += r[7 + k];
+                    }
+                    if (NN) nn_merge_f(A[q].best, A[q].addr, r[14], __float_as_int(r[15]));
+                }
+            }
+        }
+    }
+    if (js != 0) return;
+#pragma unroll
+    for (int q = 0; q < IPL; q++) {
+        const int i = ibase + il + q * IBL;
+        if (i >= p.ni) continue;
+        const int S = p.ostride;
+        if (gridDim.y == 1) {
+#pragma unroll
+            for (int k = 0; k < 7; k++) p.out[k * S + i] = double(D[q].h[k]) + double(D[q].l[k]);
+            if (NN) p.nn[i] = A[q].addr >= 0 ? p.jid[A[q].addr] : -1;
+        } else {
+            // partial record: (hi, lo) float pairs packed into the double slots
+            float2 *r = reinterpret_cast<float2 *>(p.part + size_t(blockIdx.y) * kPartFields * S);
+#pragma unroll
+            for (int k = 0; k < 7; k++) r[k * S + i] = make_float2(D[q].h[k], D[q].l[k]);
+            r[7 * S + i] = make_float2(NN ? A[q].best : __builtin_inff(), 0.0f);
+            r[8 * S + i] = make_float2(__int_as_float(NN ? A[q].addr : -1), 0.0f);
+        }
+    }
+}
+
+// ------------------------------------------------------------------------
+// Ordered reduction of the per-chunk partials (one lane per i).
+// ------------------------------------------------------------------------
+__global__ void __launch_bounds__(256) k_reduce_f64(const double *__restrict__ part, int nchunk,
+                                                    int ni, int S, const int *__restrict__ jid,
+                                                    double *__restrict__ out, int *__restrict__ nn,
+                                                    int want_nn)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= ni) return;
+    double h[7], l[7];
+#pragma unroll
+    for (int k = 0; k < 7; k++) { h[k] = 0.0; l[k] = 0.0; }
+    double best = __builtin_inf();
+    int addr = -1;
+    for (int ch = 0; ch < nchunk; ch++) {
+        const double *r = part + size_t(ch) * kPartFieldsD * S;
+#pragma unroll
+        for (int k = 0; k < 7; k++) {
+            two_sum_add_d(h[k], l[k], r[(2 * k) * S + i]);
+            l[k] += r[(2 * k + 1) * S + i];
+        }
+        if (want_nn) nn_merge(best, addr, r[14 * S + i], int(r[15 * S + i]));
+    }
+#pragma unroll
+    for (int k = 0; k < 7; k++) out[k * S + i] = h[k] + l[k];
+    if (want_nn) nn[i] = addr >= 0 ? jid[addr] : -1;
+}
+
+__global__ void __launch_bounds__(256) k_reduce_mx(const double *__restrict__ part, int nchunk,
+                                                   int ni, int S, const int *__restrict__ jid,
+                                                   double *__restrict__ out, int *__restrict__ nn,
+                                                   int want_nn)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= ni) return;
+    float h[7], l[7];
+#pragma unroll
+    for (int k = 0; k < 7; k++) { h[k] = 0.0f; l[k] = 0.0f; }
+    float best = __builtin_inff();
+    int addr = -1;
+    for (int ch = 0; ch < nchunk; ch++) {
+        const float2 *r = reinterpret_cast<const float2 *>(part + size_t(ch) * kPartFields * S);
+#pragma unroll
+        for (int k = 0; k < 7; k++) {
+            const float2 v = r[k * S + i];
+            two_sum_add(h[k], l[k], v.x);
+            l[k] += v.y;
+        }
+        if (want_nn) nn_merge_f(best, addr, r[7 * S + i].x, __float_as_int(r[8 * S + i].x));
+    }
+#pragma unroll
+    for (int k = 0; k < 7; k++) out[k * S + i] = double(h[k]) + double(l[k]);
+    if (want_nn) nn[i] = addr >= 0 ? jid[addr] : -1;
+}
+
+// ------------------------------------------------------------------------
+// Launchers
+// ------------------------------------------------------------------------
+static inline int nblocks(int n, int b) { return (n + b - 1) / b; }
+
+hipError_t launch_reset(const JStore &s, int lo, int hi, hipStream_t st)
+{
+    if (hi <= lo) return hipSuccess;
+    hipLaunchKernelGGL(k_reset, dim3(nblocks(hi - lo, 256)), dim3(256), 0, st, s, lo, hi);
+    return hipGetLastError();
+}
+
+hipError_t launch_copy_store(const JStore &dst, const JStore &src, int n, hipStream_t st)
+{
+    if (n <= 0) return hipSuccess;
+    hipLaunchKernelGGL(k_copy_store, dim3(nblocks(n, 256)), dim3(256), 0, st, dst, src, n);
+    return hipGetLastError();
+}
+
+hipError_t launch_scatter(const JStore &s, const JRecord *rec, int n, int prec, int pred_valid,
+                          double ti, int compat, hipStream_t st)
+{
+    if (n <= 0) return hipSuccess;
+    hipLaunchKernelGGL(k_scatter, dim3(nblocks(n, 256)), dim3(256), 0, st, s, rec, n, prec,
+                       pred_valid, ti, compat);
+    return hipGetLastError();
+}
+
+hipError_t launch_predict(const JStore &s, int nj, double ti, int prec, int compat, int use_k18,
+                          hipStream_t st)
+{
+    if (nj <= 0) return hipSuccess;
+    hipLaunchKernelGGL(k_predict, dim3(nblocks(nj, 256)), dim3(256), 0, st, s, nj, ti, prec,
+                       compat, use_k18);
+    return hipGetLastError();
+}
+
+hipError_t launch_iblock_self(const JStore &s, const IBlock &b, int i0, int ni, int prec,
+                              hipStream_t st)
+{
+    if (ni <= 0) return hipSuccess;
+    hipLaunchKernelGGL(k_iblock_self, dim3(nblocks(ni, 256)), dim3(256), 0, st, s, b, i0, ni,
+                       prec);
+    return hipGetLastError();
+}
+
+bool force_cfg_supported(int njs, int ipl)
+{
+    if (ipl == 1) return njs == 1 || njs == 2 || njs == 4 || njs == 8 || njs == 16 ||
+                         njs == 32 || njs == 64 || njs == 128 || njs == 256;
+    if (ipl == 2) return njs == 1 || njs == 2 || njs == 4 || njs == 8;
+    return false;
+}
+
+template <int NJS, int IPL>
+static void launch_force_t(const FParams &fp, int prec, bool nn, dim3 grid, hipStream_t st)
+{
+    if (prec == kF64) {
+        if (nn) hipLaunchKernelGGL((k_force_f64<NJS, IPL, true>), grid, dim3(kWG), 0, st, fp);
+        else hipLaunchKernelGGL((k_force_f64<NJS, IPL, false>), grid, dim3(kWG), 0, st, fp);
+    } else {
+        if (nn) hipLaunchKernelGGL((k_force_mx<NJS, IPL, true>), grid, dim3(kWG), 0, st, fp);
+        else hipLaunchKernelGGL((k_force_mx<NJS, IPL, false>), grid, dim3(kWG), 0, st, fp);
+    }
+}
+
+hipError_t launch_force(const JStore &s, const IBlock &b, int ni, int nj, double eps2, int prec,
+                        bool want_nn, const ForceCfg &cfg, hipStream_t st)
+{
+    if (ni <= 0) return hipSuccess;
+    FParams fp{};
+    fp.ix = b.x[0]; fp.iy = b.x[1]; fp.iz = b.x[2];
+    fp.ivx = b.v[0]; fp.ivy = b.v[1]; fp.ivz = b.v[2];
+    fp.iid = b.id;
+    fp.ni = ni;
+    fp.jx = s.px[0]; fp.jy = s.px[1]; fp.jz = s.px[2]; fp.jm = s.pm;
+    fp.jvx = s.pv[0]; fp.jvy = s.pv[1]; fp.jvz = s.pv[2];
+    fp.hx = s.hx[0]; fp.hy = s.hx[1]; fp.hz = s.hx[2];
+    fp.lx = s.lx[0]; fp.ly = s.lx[1]; fp.lz = s.lx[2];
+    fp.fm = s.fm; fp.fvx = s.fv[0]; fp.fvy = s.fv[1]; fp.fvz = s.fv[2];
+    fp.jid = s.pid;
+    fp.nj = nj;
+    fp.jchunk = cfg.jchunk;
+    fp.eps2 = eps2;
+    fp.out = b.out;
+    fp.nn = b.nn;
+    fp.ostride = b.cap;
+    fp.part = b.part;
+    const int ib = (kWG / cfg.njs) * cfg.ipl;
+    dim3 grid(nblocks(ni, ib), cfg.nchunk);
+    const int key = cfg.njs * 10 + cfg.ipl;
+    switch (key) {
+    case 11: launch_force_t<1, 1>(fp, prec, want_nn, grid, st); break;
+    case 21: launch_force_t<2, 1>(fp, prec, want_nn, grid, st); break;
+    case 41: launch_force_t<4, 1>(fp, prec, want_nn, grid, st); break;
+    case 81: launch_force_t<8, 1>(fp, prec, want_nn, grid, st); break;
+    case 161: launch_force_t<16, 1>(fp, prec, want_nn, grid, st); break;
+    case 321: launch_force_t<32, 1>(fp, prec, want_nn, grid, st); break;
+    case 641: launch_force_t<64, 1>(fp, prec, want_nn, grid, st); break;
+    case 1281: launch_force_t<128, 1>(fp, prec, want_nn, grid, st); break;
+    case 2561: launch_force_t<256, 1>(fp, prec, want_nn, grid, st); break;
+    case 12: launch_force_t<1, 2>(fp, prec, want_nn, grid, st); break;
+    case 22: launch_force_t<2, 2>(fp, prec, want_nn, grid, st); break;
+    case 42: launch_force_t<4, 2>(fp, prec, want_nn, grid, st); break;
+    case 82: launch_force_t<8, 2>(fp, prec, want_nn, grid, st); break;
+    default: return hipErrorInvalidValue;
+    }
+    return hipGetLastError();
+}
+
+hipError_t launch_reduce(const JStore &s, const IBlock &b, int ni, int nchunk, int prec,
+                         bool want_nn, hipStream_t st)
+{
+    if (ni <= 0 || nchunk <= 1) return hipSuccess;
+    if (prec == kF64)
+        hipLaunchKernelGGL(k_reduce_f64, dim3(nblocks(ni, 256)), dim3(256), 0, st, b.part, nchunk,
+                           ni, b.cap, s.pid, b.out, b.nn, int(want_nn));
+    else
+        hipLaunchKernelGGL(k_reduce_mx, dim3(nblocks(ni, 256)), dim3(256), 0, st, b.part, nchunk,
+                           ni, b.cap, s.pid, b.out, b.nn, int(want_nn));
+    return hipGetLastError();
+}
+
+}  // namespace g6k

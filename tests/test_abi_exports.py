@@ -1,0 +1,79 @@
+"""libg6.so loads and exports every function include/*.h declares; the C
+by-value names are weak (ph4's grape.cc defines them strongly); error paths
+that need no GPU return negative codes instead of exiting."""
+import os
+import re
+import subprocess
+
+import pytest
+
+from amuse_amd import g6 as g6mod
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+HEADERS = [os.path.join(ROOT, "include", h) for h in ("g6lib.h", "g6x.h")]
+
+
+def declared_functions(path):
+    text = open(path).read()
+    text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+    text = re.sub(r"//[^\n]*", "", text)
+    names = re.findall(r"\b([A-Za-z_][A-Za-z0-9_]*)\s*\(", text)
+    skip = {"if", "defined", "extern"}
+    return sorted({n for n in names if n not in skip and (n.startswith("g6") or n in (
+        "get_device_count", "force_j_particle_send", "get_j_part_data"))})
+
+
+def dynamic_symbols():
+    out = subprocess.run(["nm", "-D", "--defined-only", g6mod.LIBPATH], capture_output=True,
+                         text=True, check=True).stdout
+    syms = {}
+    for line in out.splitlines():
+        parts = line.split()
+        if len(parts) == 3:
+            syms[parts[2]] = parts[1]
+    return syms
+
+
+def test_library_built():
+    assert os.path.exists(g6mod.LIBPATH), "run amuse_amd.build"
+
+
+@pytest.mark.parametrize("header", HEADERS)
+def test_every_declared_function_is_exported(header):
+    syms = dynamic_symbols()
+    missing = [f for f in declared_functions(header) if f not in syms]
+    assert not missing, missing
+
+
+def test_by_value_names_are_weak_and_fortran_names_strong():
+    syms = dynamic_symbols()
+    for name in ("g6_open", "g6_close", "g6_npipes", "g6_set_ti", "g6_set_j_particle",
+                 "g6calc_firsthalf", "g6calc_lasthalf", "g6calc_lasthalf2", "get_device_count"):
+        assert syms[name] == "W", (name, syms[name])
+    for name in ("g6_open_", "g6calc_firsthalf_", "g6calc_lasthalf2_", "g6_set_j_particle_"):
+        assert syms[name] == "T", (name, syms[name])
+
+
+def test_python_binding_lists_match_exports():
+    syms = dynamic_symbols()
+    for name in g6mod.ABI_SYMBOLS + g6mod.EXT_SYMBOLS:
+        assert name in syms, name
+
+
+def test_no_reference_or_oracle_dependency():
+    out = subprocess.run(["ldd", g6mod.LIBPATH], capture_output=True, text=True).stdout
+    assert "oracle" not in out and "g6ref" not in out
+    nm = subprocess.run(["nm", "-D", "--undefined-only", g6mod.LIBPATH], capture_output=True,
+                        text=True).stdout
+    assert "orc_" not in nm
+
+
+def test_calls_before_open_fail_cleanly_without_gpu():
+    lib = g6mod.load_library()
+    if lib.get_device_count() > 0:
+        pytest.skip("a GPU is visible; covered by the gpu tests")
+    assert lib.g6_open(0) < 0
+    assert lib.g6_set_ti(0, 1.0) == -2           # G6X_ERR_NOT_OPEN
+    assert b"not open" in lib.g6x_last_error()
+    assert lib.g6x_set_precision(7) == -3
+    assert lib.g6x_set_tiles(3, 1, 0) == -3

@@ -1,0 +1,254 @@
+"""Parity of the HIP library with the oracle, through the GRAPE-6 ABI.
+
+Bars (plain FP64 path):
+  * nearest-neighbour ids bit-exact, ties included (the predictor and r^2
+    round exactly like lib/g6lib/g6lib.c and ties go to the lowest slot);
+  * acc, jerk, pot per particle within 1e-12 of the 80-bit oracle relative to
+    |value|, for particles whose sum is not cancelling by more than 1e3
+    (|value| >= 1e-3 sum|term|), and within 1e-15 * sum|term| for all;
+  * index[] written back unchanged (g6lib.c:397,427).
+Mixed path: per-particle error against the 80-bit oracle stated and bounded
+in test_mixed_precision_error_bounds.
+"""
+import numpy as np
+import pytest
+
+import oracle
+from conftest import load_golden, print_case
+
+pytestmark = pytest.mark.gpu
+
+REL = 1e-12          # per-particle relative bound, plain path
+COND = 1e-15         # bound relative to sum|term|, plain path
+
+
+def upload(g6, x, v, a2, j6, k18, m, tj, ids, skip=None):
+    n = len(ids)
+    sel = np.arange(n) if skip is None else np.flatnonzero(~skip)
+    g6.set_j_particles(sel, ids[sel], tj[sel], 0.0, m[sel], k18[sel], j6[sel], a2[sel], v[sel],
+                       x[sel])
+
+
+def errors(val, exact, sums):
+    """Per-particle |d| / |exact| and |d| / sum|term| (vectors by norm)."""
+    if val.ndim == 2:
+        d = np.linalg.norm(val - exact, axis=1)
+        mag = np.linalg.norm(exact, axis=1)
+    else:
+        d = np.abs(val - exact)
+        mag = np.abs(exact)
+    with np.errstate(divide="ignore", invalid="ignore"):
+        rel = np.where(mag > 0, d / mag, d)
+        cond = np.where(sums > 0, d / sums, d)
+    return rel, cond, mag / np.where(sums > 0, sums, 1)
+
+
+def assert_plain_parity(acc, jerk, pot, exact):
+    ea, ej, ep, sa, sj, sp = exact
+    for val, ex, s, name in ((acc, ea, sa, "acc"), (jerk, ej, sj, "jerk"), (pot, ep, sp, "pot")):
+        rel, cond, ratio = errors(val, ex, s)
+        assert cond.max() <= COND, (name, cond.max())
+        ok = ratio >= 1e-3
+        if ok.any():
+            assert rel[ok].max() <= REL, (name, rel[ok].max())
+
+
+def test_golden_reference_vectors(g6):
+    """The compiled reference's outputs, reproduced with the predictor-compat
+    polynomial (the reference's own)."""
+    g6.set_predictor_compat(True)
+    for name in ["g6ref_dt0", "g6ref_dt", "g6ref_lattice", "g6ref_special", "g6ref_probe"]:
+        print_case(f"golden {name}")
+        d = load_golden(name)
+        g6.reset()
+        never = d["id_j"] == -2
+        upload(g6, d["x"], d["v"], d["a2"], d["j6"], d["k18"], d["m"], d["tj"], d["id_j"],
+               skip=never)
+        g6.set_ti(float(d["ti"]))
+        nj = int(d["nj"])
+        acc, jerk, pot, nn, idx = g6.calc(nj, d["id_i"], d["xi"], d["vi"], float(d["eps2"]))
+        assert np.array_equal(nn, d["nn"]), name
+        assert np.array_equal(idx, d["id_i"])
+        ids = d["id_j"][:nj].copy()
+        ids[ids == -2] = -1
+        xp, vp = oracle.predict(float(d["ti"]), d["tj"][:nj], d["x"][:nj], d["v"][:nj],
+                                d["a2"][:nj], d["j6"][:nj], d["k18"][:nj], compat=True)
+        exact = oracle.force_ld(d["id_i"], d["xi"], d["vi"], float(d["eps2"]), ids,
+                                d["m"][:nj], xp, vp)
+        assert_plain_parity(acc, jerk, pot, exact)
+        # the reference's own doubles are within the same bars of the exact sum
+        assert_plain_parity(d["acc"], d["jerk"], d["pot"], exact)
+        # lasthalf without nn gives the same forces
+        acc2, jerk2, pot2, _ = g6.calc(nj, d["id_i"], d["xi"], d["vi"], float(d["eps2"]), nn=False)
+        np.testing.assert_array_equal(acc2, acc)
+        np.testing.assert_array_equal(pot2, pot)
+
+
+def plummer_state(n, seed, dt_nonzero=True):
+    from amuse_amd import ic
+    rng = np.random.Generator(np.random.PCG64(seed))
+    m, x, v = ic.plummer(n, seed)
+    a2 = rng.normal(0, 0.3, (n, 3))
+    j6 = rng.normal(0, 0.3, (n, 3))
+    k18 = np.zeros((n, 3))
+    ti = 1.0
+    tj = ti - (rng.uniform(0, 1, n) * 2.0 ** -rng.integers(3, 11, n) if dt_nonzero else 0)
+    tj = np.broadcast_to(tj, (n,)).astype(float).copy()
+    ids = np.arange(1, n + 1, dtype=np.int32)
+    return dict(m=m, x=x, v=v, a2=a2, j6=j6, k18=k18, tj=tj, ti=ti, ids=ids)
+
+
+@pytest.mark.parametrize("compat", [False, True])
+def test_plummer_4096_block_sizes(g6, compat):
+    s = plummer_state(4096, 21)
+    g6.set_predictor_compat(compat)
+    upload(g6, s["x"], s["v"], s["a2"], s["j6"], s["k18"], s["m"], s["tj"], s["ids"])
+    g6.set_ti(s["ti"])
+    xp, vp = oracle.predict(s["ti"], s["tj"], s["x"], s["v"], s["a2"], s["j6"], s["k18"],
+                            compat=compat)
+    n = 4096
+    for ni in (1, 2, 7, 64, 333, 1024, 4096):
+        print_case(f"plummer4096 compat={compat} ni={ni}")
+        sel = np.linspace(0, n - 1, ni).astype(int)
+        id_i, xi, vi = s["ids"][sel], xp[sel], vp[sel]
+        acc, jerk, pot, nn, _ = g6.calc(n, id_i, xi, vi, 1e-4)
+        _, _, _, onn, _ = oracle.force(id_i, xi, vi, 1e-4, s["ids"], s["m"], xp, vp)
+        assert np.array_equal(nn, onn), ni
+        exact = oracle.force_ld(id_i, xi, vi, 1e-4, s["ids"], s["m"], xp, vp)
+        assert_plain_parity(acc, jerk, pot, exact)
+
+
+def test_tile_shapes_agree(g6):
+    s = plummer_state(3000, 22)
+    upload(g6, s["x"], s["v"], s["a2"], s["j6"], s["k18"], s["m"], s["tj"], s["ids"])
+    g6.set_ti(s["ti"])
+    xp, vp = oracle.predict(s["ti"], s["tj"], s["x"], s["v"], s["a2"], s["j6"], s["k18"])
+    sel = np.arange(0, 3000, 7)
+    id_i, xi, vi = s["ids"][sel], xp[sel], vp[sel]
+    exact = oracle.force_ld(id_i, xi, vi, 1e-4, s["ids"], s["m"], xp, vp)
+    _, _, _, onn, _ = oracle.force(id_i, xi, vi, 1e-4, s["ids"], s["m"], xp, vp)
+    shapes = [(1, 2), (1, 1), (2, 2), (2, 1), (4, 2), (4, 1), (8, 2), (8, 1), (16, 1), (32, 1),
+              (64, 1), (128, 1), (256, 1)]
+    for njs, ipl in shapes:
+        for nchunk in (1, 3, 0):
+            print_case(f"tiles njs={njs} ipl={ipl} nchunk={nchunk}")
+            g6.set_tiles(njs, ipl, nchunk)
+            acc, jerk, pot, nn, _ = g6.calc(3000, id_i, xi, vi, 1e-4)
+            assert np.array_equal(nn, onn), (njs, ipl, nchunk)
+            assert_plain_parity(acc, jerk, pot, exact)
+    g6.set_tiles(0, 0, 0)
+
+
+def test_self_block_matches_abi_path(g6):
+    s = plummer_state(2048, 23)
+    upload(g6, s["x"], s["v"], s["a2"], s["j6"], s["k18"], s["m"], s["tj"], s["ids"])
+    g6.set_ti(s["ti"])
+    xp, vp = oracle.predict(s["ti"], s["tj"], s["x"], s["v"], s["a2"], s["j6"], s["k18"])
+    acc, jerk, pot, nn, _ = g6.calc(2048, s["ids"][100:612], xp[100:612], vp[100:612], 1e-4)
+    g6.calc_self(2048, 100, 512, 1e-4, nn=True)
+    acc2, jerk2, pot2, nn2 = g6.fetch(512)
+    assert np.array_equal(nn, nn2)
+    np.testing.assert_array_equal(acc, acc2)
+    np.testing.assert_array_equal(jerk, jerk2)
+    np.testing.assert_array_equal(pot, pot2)
+
+
+def test_deterministic_repeat(g6):
+    s = plummer_state(5000, 24)
+    upload(g6, s["x"], s["v"], s["a2"], s["j6"], s["k18"], s["m"], s["tj"], s["ids"])
+    g6.set_ti(s["ti"])
+    for prec in (0, 1):
+        g6.set_precision(prec)
+        g6.calc_self(5000, 0, 5000, 1e-4)
+        r1 = g6.fetch(5000)
+        g6.calc_self(5000, 0, 5000, 1e-4)
+        r2 = g6.fetch(5000)
+        for a, b in zip(r1, r2):
+            assert np.array_equal(a, b)
+
+
+def test_edge_cases(g6):
+    s = plummer_state(64, 25)
+    upload(g6, s["x"], s["v"], s["a2"], s["j6"], s["k18"], s["m"], s["tj"], s["ids"])
+    g6.set_ti(s["ti"])
+    xi = np.zeros((3, 3))
+    vi = np.zeros((3, 3))
+    # nj = 0: nothing interacts, nn = -1 (g6lib.c:112-121)
+    acc, jerk, pot, nn, _ = g6.calc(0, [1, 2, 3], xi, vi, 1e-4)
+    assert not acc.any() and not jerk.any() and not pot.any()
+    assert list(nn) == [-1, -1, -1]
+    # ni = 0 is a no-op
+    out = g6.calc(64, np.zeros(0, np.int32), np.zeros((0, 3)), np.zeros((0, 3)), 1e-4)
+    assert out[0].shape == (0, 3)
+    # a slot far beyond the initial capacity grows the store
+    far = 300000
+    g6.set_j_particle(far, 777, s["ti"], 0.0, 0.5, [0, 0, 0], [0, 0, 0], [0, 0, 0], [0, 0, 0],
+                      [10.0, 0, 0])
+    acc, _, pot, nn, _ = g6.calc(far + 1, [-1], [[10.0, 0.0, 1.0]], [[0, 0, 0]], 0.0)
+    assert nn[0] == 777
+    xp, vp = oracle.predict(s["ti"], s["tj"], s["x"], s["v"], s["a2"], s["j6"], s["k18"])
+    p_far = -0.5 / 1.0
+    p_rest = -(s["m"] / np.linalg.norm(xp - [10.0, 0, 1.0], axis=1)).sum()
+    assert pot[0] == pytest.approx(p_far + p_rest, rel=1e-13)
+    # reset empties every slot
+    g6.reset()
+    acc, jerk, pot, nn, _ = g6.calc(far + 1, [-1], [[10.0, 0.0, 1.0]], [[0, 0, 0]], 0.0)
+    assert pot[0] == 0.0 and nn[0] == -1
+
+
+def test_error_paths(g6):
+    import ctypes as C
+    from amuse_amd.g6 import G6Error
+    lib = g6.lib
+    with pytest.raises(G6Error):
+        g6.set_j_particle(-1, 1, 0, 0, 1, [0] * 3, [0] * 3, [0] * 3, [0] * 3, [0] * 3)
+    # lasthalf without firsthalf (calc_self clears the latch)
+    g6.calc_self(0, 0, 0, 0.0)
+    idx = np.zeros(1, np.int32)
+    buf = np.zeros((1, 3))
+    D = C.POINTER(C.c_double)
+    rc = lib.g6calc_lasthalf(0, 1, 1, idx.ctypes.data_as(C.POINTER(C.c_int)), buf.ctypes.data_as(D),
+                             buf.ctypes.data_as(D), 0.0, buf.ctypes.data_as(D),
+                             buf.ctypes.data_as(D), buf.ctypes.data_as(D), buf.ctypes.data_as(D))
+    assert rc == -4
+    g6.close()
+    assert lib.g6_set_ti(0, 1.0) == -2
+    # bad device id
+    assert lib.g6_open(99) < 0
+    g6.open()
+
+
+def test_open_close_cycles_reset_state(g6):
+    s = plummer_state(128, 26)
+    for cycle in range(3):
+        upload(g6, s["x"], s["v"], s["a2"], s["j6"], s["k18"], s["m"], s["tj"], s["ids"])
+        g6.set_ti(s["ti"])
+        acc, jerk, pot, nn, _ = g6.calc(128, s["ids"][:4], s["x"][:4], s["v"][:4], 1e-4)
+        assert (nn > 0).all()
+        g6.close()
+        g6.open()
+        acc, jerk, pot, nn, _ = g6.calc(128, s["ids"][:4], s["x"][:4], s["v"][:4], 1e-4)
+        assert not pot.any() and (nn == -1).all()
+
+
+def test_mixed_precision_error_bounds(g6):
+    """Mixed path vs the 80-bit oracle: per-particle error relative to
+    sum|term| below 2e-6, median below 2e-7; nn agrees except near-ties."""
+    s = plummer_state(8192, 27)
+    upload(g6, s["x"], s["v"], s["a2"], s["j6"], s["k18"], s["m"], s["tj"], s["ids"])
+    g6.set_ti(s["ti"])
+    g6.set_precision(1)
+    xp, vp = oracle.predict(s["ti"], s["tj"], s["x"], s["v"], s["a2"], s["j6"], s["k18"])
+    sel = np.arange(0, 8192, 5)
+    id_i, xi, vi = s["ids"][sel], xp[sel], vp[sel]
+    acc, jerk, pot, nn, _ = g6.calc(8192, id_i, xi, vi, 1e-4)
+    ea, ej, ep, sa, sj, sp = oracle.force_ld(id_i, xi, vi, 1e-4, s["ids"], s["m"], xp, vp)
+    _, _, _, onn, or2 = oracle.force(id_i, xi, vi, 1e-4, s["ids"], s["m"], xp, vp)
+    stats = {}
+    for val, ex, sm, name in ((acc, ea, sa, "acc"), (jerk, ej, sj, "jerk"), (pot, ep, sp, "pot")):
+        rel, cond, _ = errors(val, ex, sm)
+        stats[name] = (cond.max(), np.median(cond), np.median(rel))
+        assert cond.max() < 2e-6, (name, cond.max())
+        assert np.median(cond) < 2e-7, (name, np.median(cond))
+    print("mixed error (max cond, median cond, median rel):", stats)
+    assert (nn == onn).mean() > 0.99
