@@ -26,6 +26,10 @@ DEVICE_FLAGS = ["-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops"]
 LIBS = {
     "libg6.so": ["g6_kernels.hip", "g6_abi.hip"],
 }
+# host-only C++ (no device code): built with g++
+CPU_LIBS = {
+    "libhermite4.so": (["hermite4.cpp"], ["-ldl"]),
+}
 
 
 def hipcc() -> str:
@@ -64,8 +68,19 @@ def build_lib(name: str, isa: bool = False) -> str:
     return out
 
 
+def build_cpu_lib(name: str) -> str:
+    os.makedirs(LIBDIR, exist_ok=True)
+    srcs, libs = CPU_LIBS[name]
+    out = os.path.join(LIBDIR, name)
+    tmp = out + ".tmp"
+    _run(["g++", "-O2", "-std=c++17", "-fPIC", "-shared", "-fvisibility=hidden", "-Wall",
+          *[os.path.join(CSRC, s) for s in srcs], *libs, "-o", tmp])
+    os.replace(tmp, out)
+    return out
+
+
 def build_all(isa: bool = False) -> list[str]:
-    return [build_lib(n, isa) for n in LIBS]
+    return [build_lib(n, isa) for n in LIBS] + [build_cpu_lib(n) for n in CPU_LIBS]
 
 
 if __name__ == "__main__":

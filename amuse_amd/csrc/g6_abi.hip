@@ -40,7 +40,7 @@ constexpr long long kPartBudget = 1ll << 21;  // nchunk * ni limit for partials
 
 struct TuneEntry {
     int ni_lo, ni_hi;  // bucket [lo, hi]
-    int njs, ipl, nchunk;
+    int njs, ipl, nchunk, smem;
 };
 
 struct Unit {
@@ -89,8 +89,14 @@ struct Unit {
     bool launched_nn = false;
     int last_ni = 0;  // rows valid in ib.out
 
-    int tile_njs = 0, tile_ipl = 0, tile_nchunk = 0;
+    int tile_njs = 0, tile_ipl = 0, tile_nchunk = 0, tile_smem = 0;
     std::vector<TuneEntry> tune;
+
+    // neighbour lists of the last calc
+    int *d_ngb_counts = nullptr, *d_ngb_lists = nullptr;
+    int ngb_cap_i = 0;
+    std::vector<int> h_ngb_counts, h_ngb_lists;
+    int ngb_ni = 0;
 
     bool timing = false;
     struct Ev {
@@ -318,12 +324,15 @@ int ensure_predicted()
 ForceCfg choose_cfg(int ni, int nj)
 {
     ForceCfg c;
-    int njs = 0, ipl = 0, nchunk = 0;
+    int njs = 0, ipl = 0, nchunk = 0, smem = 0;
     if (U.tile_njs > 0) {
-        njs = U.tile_njs; ipl = U.tile_ipl; nchunk = U.tile_nchunk;
+        njs = U.tile_njs; ipl = U.tile_ipl; nchunk = U.tile_nchunk; smem = U.tile_smem;
     } else {
         for (const auto &t : U.tune)
-            if (ni >= t.ni_lo && ni <= t.ni_hi) { njs = t.njs; ipl = t.ipl; nchunk = t.nchunk; break; }
+            if (ni >= t.ni_lo && ni <= t.ni_hi) {
+                njs = t.njs; ipl = t.ipl; nchunk = t.nchunk; smem = t.smem;
+                break;
+            }
     }
     if (njs == 0) {
         // Largest i-block that does not leave lanes idle, then enough
@@ -338,6 +347,7 @@ ForceCfg choose_cfg(int ni, int nj)
         }
         if (njs == 0) { njs = 256; ipl = 1; }
         nchunk = 0;
+        smem = njs == 1 ? 1 : 0;
     }
     const int ib = (g6k::kWG / njs) * ipl;
     const int iblocks = (ni + ib - 1) / ib;
@@ -353,6 +363,7 @@ ForceCfg choose_cfg(int ni, int nj)
     c.ipl = ipl;
     c.nchunk = nchunk;
     c.jchunk = jchunk;
+    c.smem = smem != 0 && njs == 1;
     return c;
 }
 
@@ -633,12 +644,68 @@ G6_API int g6_reset_(int *)
 
 G6_API int g6_reset_fofpga_(int *) { return 0; }
 
-G6_API int g6_read_neighbour_list_(int *) { return 0; }
-
-G6_API int g6_get_neighbour_list_(int *, int *, int *, int *n_neighbours, int[])
+// Neighbour lists (sapporo.cpp:234-272): computed on demand for the i-block
+// of the last firsthalf against the same predicted j-set.  Returns 1 when
+// some i has more than 256 neighbours (its list then holds 256 of them), 0
+// otherwise, negative on error.
+G6_API int g6_read_neighbour_list_(int *)
 {
-    if (n_neighbours) *n_neighbours = 0;
-    return 0;
+    int rc = check_open();
+    if (rc) return rc;
+    if (!U.latched) return fail(G6X_ERR_SEQUENCE, "read_neighbour_list without a calc");
+    const int ni = U.l_ni;
+    U.ngb_ni = 0;
+    if (ni <= 0) return 0;
+    if (ni > U.ngb_cap_i) {
+        HIPCHK(hipStreamSynchronize(U.stream), "ngb sync");
+        if (U.d_ngb_counts) hipFree(U.d_ngb_counts);
+        if (U.d_ngb_lists) hipFree(U.d_ngb_lists);
+        U.d_ngb_counts = nullptr;
+        U.d_ngb_lists = nullptr;
+        U.ngb_cap_i = 0;
+        int cap = std::max(ni, 1024);
+        hipError_t e = hipMalloc(reinterpret_cast<void **>(&U.d_ngb_counts), sizeof(int) * size_t(cap));
+        if (e == hipSuccess)
+            e = hipMalloc(reinterpret_cast<void **>(&U.d_ngb_lists), sizeof(int) * size_t(cap) * g6k::kNgbCap);
+        if (e != hipSuccess) return fail(G6X_ERR_NOMEM, "hipMalloc neighbour lists", e);
+        U.ngb_cap_i = cap;
+    }
+    HIPCHK(hipMemsetAsync(U.d_ngb_counts, 0, sizeof(int) * size_t(ni), U.stream), "ngb clear");
+    HIPCHK(g6k::launch_ngb(U.js, U.ib, ni, U.l_nj, U.prec, U.d_ngb_counts, U.d_ngb_lists, U.stream), "ngb kernel");
+    U.h_ngb_counts.resize(ni);
+    U.h_ngb_lists.resize(size_t(ni) * g6k::kNgbCap);
+    HIPCHK(hipMemcpyAsync(U.h_ngb_counts.data(), U.d_ngb_counts, sizeof(int) * size_t(ni), hipMemcpyDeviceToHost, U.stream), "ngb D2H");
+    HIPCHK(hipMemcpyAsync(U.h_ngb_lists.data(), U.d_ngb_lists, sizeof(int) * size_t(ni) * g6k::kNgbCap, hipMemcpyDeviceToHost, U.stream), "ngb D2H");
+    HIPCHK(hipStreamSynchronize(U.stream), "ngb sync");
+    U.ngb_ni = ni;
+    int overflow = 0;
+    for (int i = 0; i < ni; i++) {
+        const int c = U.h_ngb_counts[i];
+        int *row = &U.h_ngb_lists[size_t(i) * g6k::kNgbCap];
+        std::sort(row, row + std::min(c, g6k::kNgbCap));
+        if (c > g6k::kNgbCap) overflow = 1;
+    }
+    return overflow;
+}
+
+// Copies the sorted neighbour ids of i-pipe *ipipe (at most *maxlength) and
+// sets *n_neighbours to the full count; returns 1 when ids were left out
+// (more than *maxlength, or more than the 256 kept), else 0.  The reference
+// (sapporo.cpp:248-272) exits on a bad ipipe; this returns G6X_ERR_ARG.
+G6_API int g6_get_neighbour_list_(int *, int *ipipe, int *maxlength, int *n_neighbours,
+                                  int list[])
+{
+    int rc = check_open();
+    if (rc) return rc;
+    if (*ipipe < 0 || *ipipe >= U.ngb_ni) return fail(G6X_ERR_ARG, "ipipe outside the last neighbour read");
+    if (*maxlength < 0) return fail(G6X_ERR_ARG, "negative maxlength");
+    const int c = U.h_ngb_counts[*ipipe];
+    const int kept = std::min(c, g6k::kNgbCap);
+    const int ncopy = std::min(kept, *maxlength);
+    const int *row = &U.h_ngb_lists[size_t(*ipipe) * g6k::kNgbCap];
+    std::copy(row, row + ncopy, list);
+    *n_neighbours = c;
+    return (c > *maxlength || c > g6k::kNgbCap) ? 1 : 0;
 }
 
 // ===========================================================================
@@ -793,16 +860,18 @@ G6_API int g6x_get_timing(double *force_ms, double *reduce_ms, double *predict_m
     return 0;
 }
 
-G6_API int g6x_set_tiles(int njs, int ipl, int nchunk)
+G6_API int g6x_set_tiles(int njs, int ipl, int nchunk, int smem)
 {
     if (njs == 0) {
-        U.tile_njs = U.tile_ipl = U.tile_nchunk = 0;
+        U.tile_njs = U.tile_ipl = U.tile_nchunk = U.tile_smem = 0;
         return 0;
     }
-    if (!g6k::force_cfg_supported(njs, ipl) || nchunk < 0) return fail(G6X_ERR_ARG, "tile config");
+    if (!g6k::force_cfg_supported(njs, ipl) || nchunk < 0 || (smem && (njs != 1 || ipl > 2)))
+        return fail(G6X_ERR_ARG, "tile config");
     U.tile_njs = njs;
     U.tile_ipl = ipl;
     U.tile_nchunk = nchunk;
+    U.tile_smem = smem ? 1 : 0;
     return 0;
 }
 
@@ -831,12 +900,16 @@ G6_API int g6x_autotune(int nj, int ni_max, double eps2)
     int rc = check_open();
     if (rc) return rc;
     if (nj <= 0 || ni_max <= 0 || ni_max > nj) return fail(G6X_ERR_ARG, "autotune sizes");
-    const int save_njs = U.tile_njs, save_ipl = U.tile_ipl, save_nc = U.tile_nchunk;
+    const int save_njs = U.tile_njs, save_ipl = U.tile_ipl, save_nc = U.tile_nchunk,
+              save_smem = U.tile_smem;
     const bool save_timing = U.timing;
     U.timing = false;
     U.tune.clear();
-    static const int kShapes[][2] = {{1, 2}, {1, 1}, {2, 2}, {2, 1}, {4, 2}, {4, 1}, {8, 2},
-                                     {8, 1}, {16, 1}, {32, 1}, {64, 1}, {128, 1}, {256, 1}};
+    // {njs, ipl, smem}
+    static const int kShapes[][3] = {{1, 2, 1}, {1, 1, 1}, {1, 2, 0}, {1, 1, 0}, {2, 2, 0},
+                                     {2, 1, 0}, {4, 2, 0}, {4, 1, 0}, {8, 2, 0}, {8, 1, 0},
+                                     {16, 1, 0}, {32, 1, 0}, {64, 1, 0}, {128, 1, 0},
+                                     {256, 1, 0}};
     hipEvent_t a = nullptr, b = nullptr;
     HIPCHK(hipEventCreate(&a), "event");
     HIPCHK(hipEventCreate(&b), "event");
@@ -844,12 +917,13 @@ G6_API int g6x_autotune(int nj, int ni_max, double eps2)
     std::vector<TuneEntry> table;
     for (int ni = 1; ni <= ni_max; ni *= 2) {
         float best_ms = 1e30f;
-        TuneEntry best{ni, ni, 0, 0, 0};
+        TuneEntry best{ni, ni, 0, 0, 0, 0};
         for (const auto &sh : kShapes) {
             const int ib = (g6k::kWG / sh[0]) * sh[1];
             if (ib > 4 * ni && ib > 1) continue;  // mostly idle lanes
             U.tile_njs = sh[0];
             U.tile_ipl = sh[1];
+            U.tile_smem = sh[2];
             U.tile_nchunk = 0;
             const int auto_nc = choose_cfg(ni, nj).nchunk;
             const int ncs[3] = {auto_nc, std::max(1, auto_nc / 2), auto_nc * 2};
@@ -874,6 +948,7 @@ G6_API int g6x_autotune(int nj, int ni_max, double eps2)
                     best_ms = ms;
                     best.njs = sh[0];
                     best.ipl = sh[1];
+                    best.smem = sh[2];
                     best.nchunk = choose_cfg(ni, nj).nchunk;
                 }
             }
@@ -891,6 +966,7 @@ done:
     U.tile_njs = save_njs;
     U.tile_ipl = save_ipl;
     U.tile_nchunk = save_nc;
+    U.tile_smem = save_smem;
     U.timing = save_timing;
     if (rc) return rc;
     U.tune = table;
@@ -901,8 +977,9 @@ G6_API int g6x_save_tuning(const char *path)
 {
     FILE *f = fopen(path, "w");
     if (!f) return fail(G6X_ERR_ARG, "cannot write tuning file");
-    fprintf(f, "# ni_lo ni_hi njs ipl nchunk\n");
-    for (const auto &t : U.tune) fprintf(f, "%d %d %d %d %d\n", t.ni_lo, t.ni_hi, t.njs, t.ipl, t.nchunk);
+    fprintf(f, "# ni_lo ni_hi njs ipl nchunk smem\n");
+    for (const auto &t : U.tune)
+        fprintf(f, "%d %d %d %d %d %d\n", t.ni_lo, t.ni_hi, t.njs, t.ipl, t.nchunk, t.smem);
     fclose(f);
     return int(U.tune.size());
 }
@@ -916,8 +993,10 @@ G6_API int g6x_load_tuning(const char *path)
     while (fgets(line, sizeof line, f)) {
         if (line[0] == '#') continue;
         TuneEntry t;
-        if (sscanf(line, "%d %d %d %d %d", &t.ni_lo, &t.ni_hi, &t.njs, &t.ipl, &t.nchunk) == 5 &&
-            g6k::force_cfg_supported(t.njs, t.ipl) && t.ni_lo <= t.ni_hi && t.nchunk >= 0)
+        if (sscanf(line, "%d %d %d %d %d %d", &t.ni_lo, &t.ni_hi, &t.njs, &t.ipl, &t.nchunk,
+                   &t.smem) == 6 &&
+            g6k::force_cfg_supported(t.njs, t.ipl) && t.ni_lo <= t.ni_hi && t.nchunk >= 0 &&
+            (!t.smem || (t.njs == 1 && t.ipl <= 2)))
             table.push_back(t);
     }
     fclose(f);

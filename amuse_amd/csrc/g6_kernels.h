@@ -29,6 +29,7 @@ constexpr int kTJ = 256;        // j per LDS tile
 constexpr int kPartFields = 9;   // mixed: (hi,lo) float pairs of acc3 jerk3 pot, r2, slot
 constexpr int kPartFieldsD = 16; // plain: hi, lo of acc3 jerk3 pot, r2, slot
 constexpr double kFar = 1.0e150;  // position given to empty slots
+constexpr int kNgbCap = 256;      // neighbour ids kept per i (sapporo NGB_PP)
 
 struct JStore {
     int cap = 0;
@@ -67,6 +68,7 @@ struct ForceCfg {
     int ipl = 1;     // i-particles per lane
     int nchunk = 1;  // workgroups along j
     int jchunk = 0;  // j per chunk (multiple of kTJ)
+    bool smem = false;  // njs == 1 only: j read through scalar loads, no LDS
 };
 
 size_t jstore_bytes(int cap);
@@ -91,5 +93,11 @@ hipError_t launch_force(const JStore &s, const IBlock &b, int ni, int nj,
                         const ForceCfg &cfg, hipStream_t st);
 hipError_t launch_reduce(const JStore &s, const IBlock &b, int ni,
                          int nchunk, int prec, bool want_nn, hipStream_t st);
+// Neighbour query over the latched i-block: ids of j with unsoftened
+// r^2 <= h2[i] (self and empty slots excluded).  counts[i] is the full count
+// (must be zeroed first); at most kNgbCap ids per i are stored, in no
+// particular order.
+hipError_t launch_ngb(const JStore &s, const IBlock &b, int ni, int nj, int prec,
+                      int *counts, int *lists, hipStream_t st);
 
 }  // namespace g6k

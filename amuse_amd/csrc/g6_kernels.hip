@@ -24,6 +24,7 @@
 // j-split.
 #include "g6_kernels.h"
 
+#include <algorithm>
 #include <cfloat>
 
 namespace g6k {
@@ -724,6 +725,128 @@ __global__ void __launch_bounds__(kWG) k_force_mx(FParams p)
 }
 
 // ------------------------------------------------------------------------
+// Scalar-load variants (one lane per i, NJS = 1): every lane of a wave needs
+// the same j, so the j record is read with wave-uniform addresses, which the
+// compiler turns into scalar (SMEM) loads into SGPRs that the VALU reads
+// directly.  No LDS traffic, no barriers in the hot loop; the workgroup's
+// four waves each hold their own 64 (or 128) i-particles.
+// ------------------------------------------------------------------------
+template <int IPL, bool NN>
+__global__ void __launch_bounds__(kWG) k_force_f64_s(FParams p)
+{
+    constexpr int IB = kWG * IPL;
+    const int t = threadIdx.x;
+    const int ibase = blockIdx.x * IB;
+    ID I[IPL];
+    AccD A[IPL];
+    DDAcc D[IPL];
+#pragma unroll
+    for (int q = 0; q < IPL; q++) {
+        const int i = min(ibase + t + q * kWG, p.ni - 1);
+        I[q].x = p.ix[i]; I[q].y = p.iy[i]; I[q].z = p.iz[i];
+        I[q].vx = p.ivx[i]; I[q].vy = p.ivy[i]; I[q].vz = p.ivz[i];
+        I[q].id = p.iid[i];
+        A[q] = AccD{0, 0, 0, 0, 0, 0, 0, __builtin_inf(), -1};
+#pragma unroll
+        for (int k = 0; k < 7; k++) { D[q].h[k] = 0.0; D[q].l[k] = 0.0; }
+    }
+    const double eps2 = p.eps2;
+    const int j0 = blockIdx.y * p.jchunk;
+    const int j1 = min(p.nj, j0 + p.jchunk);
+    for (int jb = j0; jb < j1; jb += kTJ) {
+        const int je = min(j1, jb + kTJ);
+#pragma unroll 4
+        for (int j = jb; j < je; j++) {
+            const double xj = p.jx[j], yj = p.jy[j], zj = p.jz[j], mj = p.jm[j];
+            const double vxj = p.jvx[j], vyj = p.jvy[j], vzj = p.jvz[j];
+            const int idj = p.jid[j];
+#pragma unroll
+            for (int q = 0; q < IPL; q++)
+                pair_f64<NN>(I[q], A[q], xj, yj, zj, mj, vxj, vyj, vzj, idj, j, eps2);
+        }
+#pragma unroll
+        for (int q = 0; q < IPL; q++) flush_f64(A[q], D[q]);
+    }
+#pragma unroll
+    for (int q = 0; q < IPL; q++) {
+        const int i = ibase + t + q * kWG;
+        if (i >= p.ni) continue;
+        const int S = p.ostride;
+        if (gridDim.y == 1) {
+#pragma unroll
+            for (int k = 0; k < 7; k++) p.out[k * S + i] = D[q].h[k] + D[q].l[k];
+            if (NN) p.nn[i] = A[q].addr >= 0 ? p.jid[A[q].addr] : -1;
+        } else {
+            double *r = p.part + size_t(blockIdx.y) * kPartFieldsD * S;
+#pragma unroll
+            for (int k = 0; k < 7; k++) {
+                r[(2 * k) * S + i] = D[q].h[k];
+                r[(2 * k + 1) * S + i] = D[q].l[k];
+            }
+            r[14 * S + i] = NN ? A[q].best : __builtin_inf();
+            r[15 * S + i] = NN ? double(A[q].addr) : -1.0;
+        }
+    }
+}
+
+template <int IPL, bool NN>
+__global__ void __launch_bounds__(kWG) k_force_mx_s(FParams p)
+{
+    constexpr int IB = kWG * IPL;
+    const int t = threadIdx.x;
+    const int ibase = blockIdx.x * IB;
+    IF I[IPL];
+    AccF A[IPL];
+    DSAcc D[IPL];
+#pragma unroll
+    for (int q = 0; q < IPL; q++) {
+        const int i = min(ibase + t + q * kWG, p.ni - 1);
+        const double x = p.ix[i], y = p.iy[i], z = p.iz[i];
+        I[q].hx = float(x); I[q].lx = float(x - double(I[q].hx));
+        I[q].hy = float(y); I[q].ly = float(y - double(I[q].hy));
+        I[q].hz = float(z); I[q].lz = float(z - double(I[q].hz));
+        I[q].vx = float(p.ivx[i]); I[q].vy = float(p.ivy[i]); I[q].vz = float(p.ivz[i]);
+        I[q].id = p.iid[i];
+        A[q] = AccF{0, 0, 0, 0, 0, 0, 0, __builtin_inff(), -1};
+#pragma unroll
+        for (int k = 0; k < 7; k++) { D[q].h[k] = 0.0f; D[q].l[k] = 0.0f; }
+    }
+    const float eps2 = float(p.eps2);
+    const int j0 = blockIdx.y * p.jchunk;
+    const int j1 = min(p.nj, j0 + p.jchunk);
+    for (int jb = j0; jb < j1; jb += kFlush) {
+        const int je = min(j1, jb + kFlush);
+#pragma unroll 4
+        for (int j = jb; j < je; j++) {
+            const float4 a = make_float4(p.hx[j], p.hy[j], p.hz[j], p.fm[j]);
+            const float4 b = make_float4(p.lx[j], p.ly[j], p.lz[j], p.fvx[j]);
+            const float4 c = make_float4(p.fvy[j], p.fvz[j], __int_as_float(p.jid[j]), 0.0f);
+#pragma unroll
+            for (int q = 0; q < IPL; q++) pair_mx<NN>(I[q], A[q], a, b, c, j, eps2);
+        }
+#pragma unroll
+        for (int q = 0; q < IPL; q++) flush_mx(A[q], D[q]);
+    }
+#pragma unroll
+    for (int q = 0; q < IPL; q++) {
+        const int i = ibase + t + q * kWG;
+        if (i >= p.ni) continue;
+        const int S = p.ostride;
+        if (gridDim.y == 1) {
+#pragma unroll
+            for (int k = 0; k < 7; k++) p.out[k * S + i] = double(D[q].h[k]) + double(D[q].l[k]);
+            if (NN) p.nn[i] = A[q].addr >= 0 ? p.jid[A[q].addr] : -1;
+        } else {
+            float2 *r = reinterpret_cast<float2 *>(p.part + size_t(blockIdx.y) * kPartFields * S);
+#pragma unroll
+            for (int k = 0; k < 7; k++) r[k * S + i] = make_float2(D[q].h[k], D[q].l[k]);
+            r[7 * S + i] = make_float2(NN ? A[q].best : __builtin_inff(), 0.0f);
+            r[8 * S + i] = make_float2(__int_as_float(NN ? A[q].addr : -1), 0.0f);
+        }
+    }
+}
+
+// ------------------------------------------------------------------------
 // Ordered reduction of the per-chunk partials (one lane per i).
 // ------------------------------------------------------------------------
 __global__ void __launch_bounds__(256) k_reduce_f64(const double *__restrict__ part, int nchunk,
@@ -780,6 +903,43 @@ __global__ void __launch_bounds__(256) k_reduce_mx(const double *__restrict__ pa
 }
 
 // ------------------------------------------------------------------------
+// Neighbour query (lib/sapporo_light/dev_evaluate_gravity.cu:60-67 criterion
+// r^2 <= h2 with self excluded; empty slots excluded as in g6lib.c:138-140).
+// One lane per (i, j) pair inside a 64 x 4 tile: the i-index is fast across
+// the wave so the j data is a broadcast; r^2 is rounded exactly like the
+// reference's so boundary cases agree with it.
+// ------------------------------------------------------------------------
+__global__ void __launch_bounds__(256) k_ngb(JStore s, IBlock b, int ni, int nj, int prec,
+                                             int jper, int *__restrict__ counts,
+                                             int *__restrict__ lists)
+{
+    const int i = blockIdx.x * 64 + (threadIdx.x & 63);
+    if (i >= ni) return;
+    const int jsub = threadIdx.x >> 6;
+    const double xi = b.x[0][i], yi = b.x[1][i], zi = b.x[2][i], h2 = b.h2[i];
+    const int idi = b.id[i];
+    const int j0 = blockIdx.y * jper;
+    const int j1 = min(nj, j0 + jper);
+    for (int j = j0 + jsub; j < j1; j += 4) {
+        const int idj = s.pid[j];
+        if (idj == -1 || idj == idi) continue;
+        double xj, yj, zj;
+        if (prec == kF64) {
+            xj = s.px[0][j]; yj = s.px[1][j]; zj = s.px[2][j];
+        } else {
+            xj = double(s.hx[0][j]) + double(s.lx[0][j]);
+            yj = double(s.hx[1][j]) + double(s.lx[1][j]);
+            zj = double(s.hx[2][j]) + double(s.lx[2][j]);
+        }
+        const double r2 = exact_r2(xj - xi, yj - yi, zj - zi);
+        if (r2 <= h2) {
+            const int slot = atomicAdd(&counts[i], 1);
+            if (slot < kNgbCap) lists[size_t(i) * kNgbCap + slot] = idj;
+        }
+    }
+}
+
+// ------------------------------------------------------------------------
 // Launchers
 // ------------------------------------------------------------------------
 static inline int nblocks(int n, int b) { return (n + b - 1) / b; }
@@ -825,12 +985,37 @@ hipError_t launch_iblock_self(const JStore &s, const IBlock &b, int i0, int ni, 
     return hipGetLastError();
 }
 
+hipError_t launch_ngb(const JStore &s, const IBlock &b, int ni, int nj, int prec, int *counts,
+                      int *lists, hipStream_t st)
+{
+    if (ni <= 0 || nj <= 0) return hipSuccess;
+    const int iblocks = nblocks(ni, 64);
+    int nchunk = std::max(1, std::min(nblocks(nj, 1024), nblocks(4096, iblocks)));
+    const int jper = nblocks(nj, nchunk);
+    nchunk = nblocks(nj, jper);
+    hipLaunchKernelGGL(k_ngb, dim3(iblocks, nchunk), dim3(256), 0, st, s, b, ni, nj, prec, jper,
+                       counts, lists);
+    return hipGetLastError();
+}
+
 bool force_cfg_supported(int njs, int ipl)
 {
     if (ipl == 1) return njs == 1 || njs == 2 || njs == 4 || njs == 8 || njs == 16 ||
                          njs == 32 || njs == 64 || njs == 128 || njs == 256;
     if (ipl == 2) return njs == 1 || njs == 2 || njs == 4 || njs == 8;
     return false;
+}
+
+template <int IPL>
+static void launch_force_s(const FParams &fp, int prec, bool nn, dim3 grid, hipStream_t st)
+{
+    if (prec == kF64) {
+        if (nn) hipLaunchKernelGGL((k_force_f64_s<IPL, true>), grid, dim3(kWG), 0, st, fp);
+        else hipLaunchKernelGGL((k_force_f64_s<IPL, false>), grid, dim3(kWG), 0, st, fp);
+    } else {
+        if (nn) hipLaunchKernelGGL((k_force_mx_s<IPL, true>), grid, dim3(kWG), 0, st, fp);
+        else hipLaunchKernelGGL((k_force_mx_s<IPL, false>), grid, dim3(kWG), 0, st, fp);
+    }
 }
 
 template <int NJS, int IPL>
@@ -869,6 +1054,13 @@ hipError_t launch_force(const JStore &s, const IBlock &b, int ni, int nj, double
     fp.part = b.part;
     const int ib = (kWG / cfg.njs) * cfg.ipl;
     dim3 grid(nblocks(ni, ib), cfg.nchunk);
+    if (cfg.smem) {
+        if (cfg.njs != 1) return hipErrorInvalidValue;
+        if (cfg.ipl == 1) launch_force_s<1>(fp, prec, want_nn, grid, st);
+        else if (cfg.ipl == 2) launch_force_s<2>(fp, prec, want_nn, grid, st);
+        else return hipErrorInvalidValue;
+        return hipGetLastError();
+    }
     const int key = cfg.njs * 10 + cfg.ipl;
     switch (key) {
     case 11: launch_force_t<1, 1>(fp, prec, want_nn, grid, st); break;

@@ -85,7 +85,7 @@ def load_library(path: str = LIBPATH) -> C.CDLL:
         "g6x_sync": (C.c_int, []),
         "g6x_timing": (C.c_int, [C.c_int]),
         "g6x_get_timing": (C.c_int, [_D, _D, _D, C.POINTER(C.c_longlong), _D]),
-        "g6x_set_tiles": (C.c_int, [C.c_int, C.c_int, C.c_int]),
+        "g6x_set_tiles": (C.c_int, [C.c_int, C.c_int, C.c_int, C.c_int]),
         "g6x_autotune": (C.c_int, [C.c_int, C.c_int, C.c_double]),
         "g6x_save_tuning": (C.c_int, [C.c_char_p]),
         "g6x_load_tuning": (C.c_int, [C.c_char_p]),
@@ -231,8 +231,9 @@ class G6:
     def set_npipes(self, n: int) -> None:
         self._check(self.lib.g6x_set_npipes(int(n)), "g6x_set_npipes")
 
-    def set_tiles(self, njs: int, ipl: int, nchunk: int = 0) -> None:
-        self._check(self.lib.g6x_set_tiles(int(njs), int(ipl), int(nchunk)), "g6x_set_tiles")
+    def set_tiles(self, njs: int, ipl: int, nchunk: int = 0, smem: bool = False) -> None:
+        self._check(self.lib.g6x_set_tiles(int(njs), int(ipl), int(nchunk), int(bool(smem))),
+                    "g6x_set_tiles")
 
     def calc_self(self, nj: int, i0: int, ni: int, eps2: float, nn: bool = True) -> None:
         self._check(self.lib.g6x_calc_self(int(nj), int(i0), int(ni), float(eps2), int(nn)),

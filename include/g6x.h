@@ -76,9 +76,10 @@ int g6x_get_timing(double *force_ms, double *reduce_ms, double *predict_ms,
 
 /* Tile configuration of the force kernel.  njs = lanes sharing one
  * i-particle (1..256, power of two), ipl = i-particles per lane (1 or 2),
- * nchunk = j-chunks (workgroups along j; 0 = automatic).  njs = 0 restores
- * the automatic choice (autotuned table, else heuristic). */
-int g6x_set_tiles(int njs, int ipl, int nchunk);
+ * nchunk = j-chunks (workgroups along j; 0 = automatic), smem = 1 reads the
+ * j-set through scalar loads instead of LDS tiles (njs = 1 only).  njs = 0
+ * restores the automatic choice (autotuned table, else heuristic). */
+int g6x_set_tiles(int njs, int ipl, int nchunk, int smem);
 /* Runs the autotuner for (ni, nj) shapes on the current j-store: times every
  * valid tile configuration and keeps the fastest per ni bucket.  Returns the
  * number of configurations timed. */

@@ -268,6 +268,27 @@ ORC_EXPORT orc_unit *orc_unit_open(int cap_j, int cap_i, int compat)
     return u;
 }
 
+/* Enlarges the j capacity to at least cap_j (new slots empty, id -1). */
+ORC_EXPORT int orc_unit_grow(orc_unit *u, int cap_j)
+{
+    if (cap_j <= u->cap_j) return 0;
+    int cap = u->cap_j > 0 ? u->cap_j : 1;
+    while (cap < cap_j) cap *= 2;
+    orc_jrec *j = (orc_jrec *)realloc(u->j, sizeof(orc_jrec) * (size_t)cap);
+    if (!j) return -1;
+    u->j = j;
+    orc_jpred *jp = (orc_jpred *)realloc(u->jp, sizeof(orc_jpred) * (size_t)cap);
+    if (!jp) return -1;
+    u->jp = jp;
+    for (int k = u->cap_j; k < cap; k++) {
+        memset(&u->j[k], 0, sizeof(orc_jrec));
+        memset(&u->jp[k], 0, sizeof(orc_jpred));
+        u->j[k].id = u->jp[k].id = -1;
+    }
+    u->cap_j = cap;
+    return 0;
+}
+
 ORC_EXPORT void orc_unit_close(orc_unit *u)
 {
     if (!u) return;

@@ -35,9 +35,9 @@ def g6(g6lib):
     g6lib.open()
     g6lib.set_precision(0)
     g6lib.set_predictor_compat(False)
-    g6lib.set_tiles(0, 0, 0)
+    g6lib.set_tiles(0, 0)
     yield g6lib
-    g6lib.set_tiles(0, 0, 0)
+    g6lib.set_tiles(0, 0)
     g6lib.set_precision(0)
     g6lib.set_predictor_compat(False)
     g6lib.close()

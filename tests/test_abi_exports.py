@@ -76,4 +76,5 @@ def test_calls_before_open_fail_cleanly_without_gpu():
     assert lib.g6_set_ti(0, 1.0) == -2           # G6X_ERR_NOT_OPEN
     assert b"not open" in lib.g6x_last_error()
     assert lib.g6x_set_precision(7) == -3
-    assert lib.g6x_set_tiles(3, 1, 0) == -3
+    assert lib.g6x_set_tiles(3, 1, 0, 0) == -3
+    assert lib.g6x_set_tiles(2, 1, 0, 1) == -3

@@ -19,7 +19,9 @@ from conftest import load_golden, print_case
 pytestmark = pytest.mark.gpu
 
 REL = 1e-12          # per-particle relative bound, plain path
-COND = 1e-15         # bound relative to sum|term|, plain path
+# bound relative to sum|term|, plain path; a jerk term cancels inside itself
+# (v/s^1.5 - 3 (r.v) r/s^2.5), so its rounding is larger relative to |term|
+COND = {"acc": 1e-15, "jerk": 4e-15, "pot": 1e-15}
 
 
 def upload(g6, x, v, a2, j6, k18, m, tj, ids, skip=None):
@@ -47,7 +49,7 @@ def assert_plain_parity(acc, jerk, pot, exact):
     ea, ej, ep, sa, sj, sp = exact
     for val, ex, s, name in ((acc, ea, sa, "acc"), (jerk, ej, sj, "jerk"), (pot, ep, sp, "pot")):
         rel, cond, ratio = errors(val, ex, s)
-        assert cond.max() <= COND, (name, cond.max())
+        assert cond.max() <= COND[name], (name, cond.max())
         ok = ratio >= 1e-3
         if ok.any():
             assert rel[ok].max() <= REL, (name, rel[ok].max())
@@ -127,16 +129,27 @@ def test_tile_shapes_agree(g6):
     id_i, xi, vi = s["ids"][sel], xp[sel], vp[sel]
     exact = oracle.force_ld(id_i, xi, vi, 1e-4, s["ids"], s["m"], xp, vp)
     _, _, _, onn, _ = oracle.force(id_i, xi, vi, 1e-4, s["ids"], s["m"], xp, vp)
-    shapes = [(1, 2), (1, 1), (2, 2), (2, 1), (4, 2), (4, 1), (8, 2), (8, 1), (16, 1), (32, 1),
-              (64, 1), (128, 1), (256, 1)]
-    for njs, ipl in shapes:
-        for nchunk in (1, 3, 0):
-            print_case(f"tiles njs={njs} ipl={ipl} nchunk={nchunk}")
-            g6.set_tiles(njs, ipl, nchunk)
-            acc, jerk, pot, nn, _ = g6.calc(3000, id_i, xi, vi, 1e-4)
-            assert np.array_equal(nn, onn), (njs, ipl, nchunk)
-            assert_plain_parity(acc, jerk, pot, exact)
-    g6.set_tiles(0, 0, 0)
+    shapes = [(1, 2, 1), (1, 1, 1), (1, 2, 0), (1, 1, 0), (2, 2, 0), (2, 1, 0), (4, 2, 0),
+              (4, 1, 0), (8, 2, 0), (8, 1, 0), (16, 1, 0), (32, 1, 0), (64, 1, 0), (128, 1, 0),
+              (256, 1, 0)]
+    for prec in (0, 1):
+        g6.set_precision(prec)
+        first = None
+        for njs, ipl, smem in shapes:
+            for nchunk in (1, 3, 0):
+                print_case(f"tiles prec={prec} njs={njs} ipl={ipl} smem={smem} nchunk={nchunk}")
+                g6.set_tiles(njs, ipl, nchunk, smem)
+                acc, jerk, pot, nn, _ = g6.calc(3000, id_i, xi, vi, 1e-4)
+                if prec == 0:
+                    assert np.array_equal(nn, onn), (njs, ipl, nchunk, smem)
+                    assert_plain_parity(acc, jerk, pot, exact)
+                else:
+                    # mixed: every shape within FP32 rounding of the first
+                    if first is None:
+                        first = (acc, pot)
+                    scale = np.abs(first[0]).max()
+                    assert np.abs(acc - first[0]).max() < 1e-5 * scale
+    g6.set_tiles(0, 0)
 
 
 def test_self_block_matches_abi_path(g6):
