@@ -347,7 +347,7 @@ ForceCfg choose_cfg(int ni, int nj)
         }
         if (njs == 0) { njs = 256; ipl = 1; }
         nchunk = 0;
-        smem = njs == 1 ? 1 : 0;
+        smem = 0;  // LDS tiles measured faster than scalar loads at every size tried
     }
     const int ib = (g6k::kWG / njs) * ipl;
     const int iblocks = (ni + ib - 1) / ib;

@@ -1,0 +1,279 @@
+// All-pairs acc/jerk on caller-owned device arrays, and its adjoint.
+//
+// Forward (the pair sum of lib/g6lib/g6lib.c:165-183, without potential):
+//   a_i = sum_j m_j r / s^1.5,  j_i = sum_j m_j (v / s^1.5 - 3 (r.v) r / s^2.5),
+//   r = x_j - x_i, v = v_j - v_i, s = r^2 + eps2, j != i.
+//
+// Adjoint: given gA_i = dL/da_i and gJ_i = dL/dj_i, with y = s^-1/2,
+//   Gr(r, v; gA, gJ) = y^3 gA - 3 y^5 (r.v) gJ - 3 y^5 (r.gJ) v
+//                      + r [-3 y^5 (r.gA + v.gJ) + 15 y^7 (r.v)(r.gJ)]
+//   Gv(r, v; gJ)     = y^3 gJ - 3 y^5 (r.gJ) r
+// are the pair's vector-Jacobian products for d/dr and d/dv (both even in
+// (r, v) -> (-r, -v)), and for particle k, summing over p != k with
+// r = x_p - x_k, v = v_p - v_k:
+//   dL/dx_k = m_k sum_p Gr(r, v; gA_p, gJ_p) - sum_p m_p Gr(r, v; gA_k, gJ_k)
+//   dL/dv_k = m_k sum_p Gv(r, v; gJ_p)       - sum_p m_p Gv(r, v; gJ_k)
+//   dL/dm_k = - sum_p [gA_p . r y^3 + gJ_p . (v y^3 - 3 (r.v) r y^5)]
+// (the m_k terms are k acting as the source of p's force, the m_p terms k
+// receiving p's force; the sign of r flips between the two views).
+//
+// Arrays are [n][3] row-major (torch's contiguous (n, 3)); T is float or
+// double.  Each workgroup covers 256/NJS i-particles, NJS lanes share one
+// i and split the j-range; j-tiles go through LDS.  Sums are compensated per
+// tile.  No allocation: the caller passes every buffer and its stream.
+#include <hip/hip_runtime.h>
+
+#define G6A_API extern "C" __attribute__((visibility("default")))
+
+namespace g6a {
+
+constexpr int kWG = 256;
+constexpr int kTJ = 256;
+
+template <class T>
+__device__ __forceinline__ T rsq(T s);
+template <>
+__device__ __forceinline__ float rsq<float>(float s)
+{
+    return __builtin_amdgcn_rsqf(s);
+}
+template <>
+__device__ __forceinline__ double rsq<double>(double s)
+{
+    const double y = __builtin_amdgcn_rsq(s);
+    const double h = s * y;
+    const double e = __builtin_fma(-h, y, 1.0);
+    return __builtin_fma(y * e, __builtin_fma(e, 0.375, 0.5), y);
+}
+
+template <class T>
+__device__ __forceinline__ void two_sum(T &hi, T &lo, T x)
+{
+    const T s = hi + x;
+    const T bp = s - hi;
+    lo += (hi - (s - bp)) + (x - bp);
+    hi = s;
+}
+
+template <class T, int NJS>
+__global__ void __launch_bounds__(kWG) k_forces(int n, const T *__restrict__ x,
+                                                const T *__restrict__ v,
+                                                const T *__restrict__ m, T eps2,
+                                                T *__restrict__ acc, T *__restrict__ jerk)
+{
+    __shared__ T sx[kTJ][3], sv[kTJ][3], sm[kTJ];
+    const int t = threadIdx.x;
+    const int js = t % NJS, il = t / NJS;
+    const int i = blockIdx.x * (kWG / NJS) + il;
+    const int ic = i < n ? i : n - 1;
+    const T xi = x[3 * ic], yi = x[3 * ic + 1], zi = x[3 * ic + 2];
+    const T vxi = v[3 * ic], vyi = v[3 * ic + 1], vzi = v[3 * ic + 2];
+    T h[6] = {0, 0, 0, 0, 0, 0}, l[6] = {0, 0, 0, 0, 0, 0};
+    for (int tb = 0; tb < n; tb += kTJ) {
+        const int jn = min(kTJ, n - tb);
+        __syncthreads();
+        if (t < jn) {
+            const int q = tb + t;
+#pragma unroll
+            for (int k = 0; k < 3; k++) {
+                sx[t][k] = x[3 * q + k];
+                sv[t][k] = v[3 * q + k];
+            }
+            sm[t] = m[q];
+        }
+        __syncthreads();
+        T s6[6] = {0, 0, 0, 0, 0, 0};
+        for (int jj = js; jj < jn; jj += NJS) {
+            const T dx = sx[jj][0] - xi, dy = sx[jj][1] - yi, dz = sx[jj][2] - zi;
+            const T s = dx * dx + dy * dy + dz * dz + eps2;
+            T y = rsq<T>(s);
+            y = (tb + jj == ic) ? T(0) : y;
+            const T y2 = y * y, mi3 = sm[jj] * y * y2;
+            const T dvx = sv[jj][0] - vxi, dvy = sv[jj][1] - vyi, dvz = sv[jj][2] - vzi;
+            const T rv = dx * dvx + dy * dvy + dz * dvz;
+            const T c = T(-3) * rv * y2;
+            s6[0] += mi3 * dx; s6[1] += mi3 * dy; s6[2] += mi3 * dz;
+            s6[3] += mi3 * (dvx + c * dx); s6[4] += mi3 * (dvy + c * dy); s6[5] += mi3 * (dvz + c * dz);
+        }
+#pragma unroll
+        for (int k = 0; k < 6; k++) two_sum(h[k], l[k], s6[k]);
+    }
+#pragma unroll
+    for (int o = 1; o < NJS; o <<= 1)
+#pragma unroll
+        for (int k = 0; k < 6; k++) {
+            const T oh = __shfl_xor(h[k], o), ol = __shfl_xor(l[k], o);
+            two_sum(h[k], l[k], oh);
+            l[k] += ol;
+        }
+    if (js == 0 && i < n) {
+#pragma unroll
+        for (int k = 0; k < 3; k++) {
+            acc[3 * i + k] = h[k] + l[k];
+            jerk[3 * i + k] = h[3 + k] + l[3 + k];
+        }
+    }
+}
+
+template <class T, int NJS>
+__global__ void __launch_bounds__(kWG) k_adjoint(int n, const T *__restrict__ x,
+                                                 const T *__restrict__ v,
+                                                 const T *__restrict__ m,
+                                                 const T *__restrict__ ga,
+                                                 const T *__restrict__ gj, T eps2,
+                                                 T *__restrict__ gx, T *__restrict__ gv,
+                                                 T *__restrict__ gm)
+{
+    __shared__ T sx[kTJ][3], sv[kTJ][3], sga[kTJ][3], sgj[kTJ][3], sm[kTJ];
+    const int t = threadIdx.x;
+    const int js = t % NJS, il = t / NJS;
+    const int k0 = blockIdx.x * (kWG / NJS) + il;
+    const int kc = k0 < n ? k0 : n - 1;
+    T X[3], V[3], GA[3], GJ[3];
+#pragma unroll
+    for (int q = 0; q < 3; q++) {
+        X[q] = x[3 * kc + q];
+        V[q] = v[3 * kc + q];
+        GA[q] = ga[3 * kc + q];
+        GJ[q] = gj[3 * kc + q];
+    }
+    // hi/lo of: self_r[3] self_v[3] other_r[3] other_v[3] dm
+    T h[13], l[13];
+#pragma unroll
+    for (int q = 0; q < 13; q++) { h[q] = 0; l[q] = 0; }
+    for (int tb = 0; tb < n; tb += kTJ) {
+        const int jn = min(kTJ, n - tb);
+        __syncthreads();
+        if (t < jn) {
+            const int p = tb + t;
+#pragma unroll
+            for (int q = 0; q < 3; q++) {
+                sx[t][q] = x[3 * p + q];
+                sv[t][q] = v[3 * p + q];
+                sga[t][q] = ga[3 * p + q];
+                sgj[t][q] = gj[3 * p + q];
+            }
+            sm[t] = m[p];
+        }
+        __syncthreads();
+        T s[13];
+#pragma unroll
+        for (int q = 0; q < 13; q++) s[q] = 0;
+        for (int jj = js; jj < jn; jj += NJS) {
+            const T r0 = sx[jj][0] - X[0], r1 = sx[jj][1] - X[1], r2 = sx[jj][2] - X[2];
+            const T w0 = sv[jj][0] - V[0], w1 = sv[jj][1] - V[1], w2 = sv[jj][2] - V[2];
+            const T ss = r0 * r0 + r1 * r1 + r2 * r2 + eps2;
+            T y = rsq<T>(ss);
+            y = (tb + jj == kc) ? T(0) : y;
+            const T y2 = y * y, y3 = y * y2, y5 = y3 * y2, y7 = y5 * y2;
+            const T rv = r0 * w0 + r1 * w1 + r2 * w2;
+            // p's gradients, weight m_k applied at the end
+            {
+                const T a0 = sga[jj][0], a1 = sga[jj][1], a2 = sga[jj][2];
+                const T b0 = sgj[jj][0], b1 = sgj[jj][1], b2 = sgj[jj][2];
+                const T rga = r0 * a0 + r1 * a1 + r2 * a2;
+                const T vgj = w0 * b0 + w1 * b1 + w2 * b2;
+                const T rgj = r0 * b0 + r1 * b1 + r2 * b2;
+                const T cr = T(-3) * y5 * (rga + vgj) + T(15) * y7 * rv * rgj;
+                const T cj = T(-3) * y5 * rv;
+                const T cv = T(-3) * y5 * rgj;
+                s[0] += y3 * a0 + cj * b0 + cv * w0 + cr * r0;
+                s[1] += y3 * a1 + cj * b1 + cv * w1 + cr * r1;
+                s[2] += y3 * a2 + cj * b2 + cv * w2 + cr * r2;
+                s[3] += y3 * b0 + cv * r0;
+                s[4] += y3 * b1 + cv * r1;
+                s[5] += y3 * b2 + cv * r2;
+                // dm_k: -(gA_p . r y3 + gJ_p . (w y3 - 3 rv r y5))
+                s[12] -= rga * y3 + vgj * y3 + cj * rgj;
+            }
+            // k's own gradients, weight m_p
+            {
+                const T mp = sm[jj];
+                const T my3 = mp * y3, my5 = mp * y5, my7 = mp * y7;
+                const T rga = r0 * GA[0] + r1 * GA[1] + r2 * GA[2];
+                const T vgj = w0 * GJ[0] + w1 * GJ[1] + w2 * GJ[2];
+                const T rgj = r0 * GJ[0] + r1 * GJ[1] + r2 * GJ[2];
+                const T cr = T(-3) * my5 * (rga + vgj) + T(15) * my7 * rv * rgj;
+                const T cj = T(-3) * my5 * rv;
+                const T cv = T(-3) * my5 * rgj;
+                s[6] += my3 * GA[0] + cj * GJ[0] + cv * w0 + cr * r0;
+                s[7] += my3 * GA[1] + cj * GJ[1] + cv * w1 + cr * r1;
+                s[8] += my3 * GA[2] + cj * GJ[2] + cv * w2 + cr * r2;
+                s[9] += my3 * GJ[0] + cv * r0;
+                s[10] += my3 * GJ[1] + cv * r1;
+                s[11] += my3 * GJ[2] + cv * r2;
+            }
+        }
+#pragma unroll
+        for (int q = 0; q < 13; q++) two_sum(h[q], l[q], s[q]);
+    }
+#pragma unroll
+    for (int o = 1; o < NJS; o <<= 1)
+#pragma unroll
+        for (int q = 0; q < 13; q++) {
+            const T oh = __shfl_xor(h[q], o), ol = __shfl_xor(l[q], o);
+            two_sum(h[q], l[q], oh);
+            l[q] += ol;
+        }
+    if (js == 0 && k0 < n) {
+        const T mk = m[k0];
+#pragma unroll
+        for (int q = 0; q < 3; q++) {
+            gx[3 * k0 + q] = mk * (h[q] + l[q]) - (h[6 + q] + l[6 + q]);
+            gv[3 * k0 + q] = mk * (h[3 + q] + l[3 + q]) - (h[9 + q] + l[9 + q]);
+        }
+        gm[k0] = h[12] + l[12];
+    }
+}
+
+template <class T>
+int launch_forces(int n, const T *x, const T *v, const T *m, T eps2, T *acc, T *jerk,
+                  hipStream_t st)
+{
+    if (n <= 0) return 0;
+    // enough workgroups for 256 CUs: split j over lanes when n is small
+    constexpr int NJS = 8;
+    const int grid = (n + kWG / NJS - 1) / (kWG / NJS);
+    hipLaunchKernelGGL((k_forces<T, NJS>), dim3(grid), dim3(kWG), 0, st, n, x, v, m, eps2, acc,
+                       jerk);
+    return hipGetLastError() == hipSuccess ? 0 : -5;
+}
+
+template <class T>
+int launch_adjoint(int n, const T *x, const T *v, const T *m, const T *ga, const T *gj, T eps2,
+                   T *gx, T *gv, T *gm, hipStream_t st)
+{
+    if (n <= 0) return 0;
+    constexpr int NJS = 8;
+    const int grid = (n + kWG / NJS - 1) / (kWG / NJS);
+    hipLaunchKernelGGL((k_adjoint<T, NJS>), dim3(grid), dim3(kWG), 0, st, n, x, v, m, ga, gj, eps2,
+                       gx, gv, gm);
+    return hipGetLastError() == hipSuccess ? 0 : -5;
+}
+
+}  // namespace g6a
+
+G6A_API int g6a_forces_f32(int n, const float *x, const float *v, const float *m, float eps2,
+                           float *acc, float *jerk, void *stream)
+{
+    return g6a::launch_forces<float>(n, x, v, m, eps2, acc, jerk, static_cast<hipStream_t>(stream));
+}
+G6A_API int g6a_forces_f64(int n, const double *x, const double *v, const double *m,
+                           double eps2, double *acc, double *jerk, void *stream)
+{
+    return g6a::launch_forces<double>(n, x, v, m, eps2, acc, jerk, static_cast<hipStream_t>(stream));
+}
+G6A_API int g6a_adjoint_f32(int n, const float *x, const float *v, const float *m,
+                            const float *ga, const float *gj, float eps2, float *gx, float *gv,
+                            float *gm, void *stream)
+{
+    return g6a::launch_adjoint<float>(n, x, v, m, ga, gj, eps2, gx, gv, gm,
+                                      static_cast<hipStream_t>(stream));
+}
+G6A_API int g6a_adjoint_f64(int n, const double *x, const double *v, const double *m,
+                            const double *ga, const double *gj, double eps2, double *gx,
+                            double *gv, double *gm, void *stream)
+{
+    return g6a::launch_adjoint<double>(n, x, v, m, ga, gj, eps2, gx, gv, gm,
+                                       static_cast<hipStream_t>(stream));
+}

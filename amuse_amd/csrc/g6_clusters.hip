@@ -1,0 +1,804 @@
+// Batched ensembles: thousands of independent small N-body clusters advanced
+// together on one GPU.
+//
+// Scheme per cluster (restating the shared-timestep Hermite code
+// src/amuse_hermite/interface.cc:266-281 predictor, :536-616 step, with the
+// step size taken from ph4's Aarseth criterion and power-of-two
+// commensurability, src/amuse_ph4/src/idata.cc:431-512, minimised over the
+// cluster as hermite shares one step across its particles):
+//
+//   predict   x_p = x + v dt + a dt^2/2 + j dt^3/6, v_p = v + a dt + j dt^2/2
+//             (FP64), stored again as double-single + FP32 for the force
+//   force     one persistent launch for every active cluster: workgroups pull
+//             (cluster, i-block) items off a queue sorted by cost; NJS lanes
+//             share an i-particle (NJS from the tuned table of the cluster's
+//             size class); j-tiles of the cluster staged in LDS; mixed
+//             (double-single dx, FP32 r^-3, compensated sums) or FP64
+//   correct   ph4's Hermite corrector and Aarseth step (FP64), atomic minimum
+//             of the step over the cluster
+//   update    t += dt, next shared step (halve once, or double when
+//             commensurate), retire clusters that reached t_end
+//
+// The four kernels of a step contain no memset and no host synchronisation,
+// so a sequence of steps is captured once into a hipGraph and replayed.
+#include "g6_kernels.h"
+#include "../../include/g6x.h"
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <numeric>
+#include <string>
+#include <vector>
+
+#define G6C_API extern "C" __attribute__((visibility("default")))
+
+namespace g6c {
+
+constexpr int kWG = 256;
+constexpr int kTJ = 256;
+constexpr int kClasses = 8;  // size classes: <=16, <=32, ..., <=2048 (and above)
+constexpr int kFlushC = 64;
+
+struct Dev {
+    int nc = 0, np = 0;
+    int *off = nullptr, *cnt = nullptr, *pc = nullptr;  // cluster offsets/counts, particle->cluster
+    double *t = nullptr, *dt = nullptr;
+    unsigned long long *dtmin = nullptr;
+    int *active = nullptr, *nsteps = nullptr;
+    double *x[3] = {}, *v[3] = {}, *a[3] = {}, *j[3] = {}, *m = nullptr;
+    double *px[3] = {}, *pv[3] = {};
+    float *hx[3] = {}, *lx[3] = {}, *fv[3] = {}, *fm = nullptr;
+    double *na[3] = {}, *nj[3] = {}, *npot = nullptr;
+    double *energy = nullptr;  // [nc]
+    int *item_c = nullptr, *item_i0 = nullptr, *item_njs = nullptr;
+    int nitems = 0;
+    int *queue = nullptr, *nactive = nullptr;
+    double t_end = 1.0, eta = 0.02, eps2 = 1e-6, dt_limit = 0.03125, dt_fac = 0.0625;
+};
+
+__device__ __forceinline__ unsigned long long dbits(double x)
+{
+    return static_cast<unsigned long long>(__double_as_longlong(x));
+}
+
+// ---------------------------------------------------------------------------
+// predictor (FP64), also resets the work queue of the force kernel
+// ---------------------------------------------------------------------------
+__global__ void __launch_bounds__(256) kc_predict(Dev d, int zero_step)
+{
+    const int p = blockIdx.x * blockDim.x + threadIdx.x;
+    if (p == 0) *d.queue = 0;
+    if (p >= d.np) return;
+    const int c = d.pc[p];
+    if (!d.active[c]) return;
+    const double dt = zero_step ? 0.0 : d.dt[c];
+#pragma unroll
+    for (int k = 0; k < 3; k++) {
+        const double x = d.x[k][p], v = d.v[k][p], a = d.a[k][p], j = d.j[k][p];
+        // src/amuse_hermite/interface.cc:276-278
+        const double xp = x + v * dt + a * dt * dt / 2 + j * dt * dt * dt / 6;
+        const double vp = v + a * dt + j * dt * dt / 2;
+        d.px[k][p] = xp;
+        d.pv[k][p] = vp;
+        const float hi = float(xp);
+        d.hx[k][p] = hi;
+        d.lx[k][p] = float(xp - double(hi));
+        d.fv[k][p] = float(vp);
+    }
+}
+
+// ---------------------------------------------------------------------------
+// persistent force kernel
+// ---------------------------------------------------------------------------
+struct TileF {
+    float4 a[kTJ];  // hx hy hz m
+    float4 b[kTJ];  // lx ly lz vx
+    float2 c[kTJ];  // vy vz
+};
+struct TileD {
+    double4 a[kTJ];  // x y z m
+    double4 b[kTJ];  // vx vy vz -
+};
+
+__device__ __forceinline__ void two_sum_f(float &hi, float &lo, float x)
+{
+    const float s = hi + x;
+    const float bp = s - hi;
+    lo += (hi - (s - bp)) + (x - bp);
+    hi = s;
+}
+__device__ __forceinline__ void two_sum_d(double &hi, double &lo, double x)
+{
+    const double s = hi + x;
+    const double bp = s - hi;
+    lo += (hi - (s - bp)) + (x - bp);
+    hi = s;
+}
+
+template <bool MIXED>
+__global__ void __launch_bounds__(kWG) kc_force(Dev d)
+{
+    __shared__ union {
+        TileF f;
+        TileD dd;
+    } tile;
+    __shared__ int s_item;
+    const int t = threadIdx.x;
+    for (;;) {
+        if (t == 0) s_item = atomicAdd(d.queue, 1);
+        __syncthreads();
+        const int it = s_item;
+        __syncthreads();
+        if (it >= d.nitems) break;
+        const int c = d.item_c[it];
+        if (!d.active[c]) continue;
+        const int njs = d.item_njs[it];
+        const int off = d.off[c], n = d.cnt[c];
+        const int js = t % njs;
+        const int il = t / njs;
+        const int i = d.item_i0[it] + il;  // local index in the cluster
+        const bool ok = i < n;
+        const int ip = off + (ok ? i : 0);
+        // accumulators: hi/lo pairs of acc3 jerk3 pot
+        float fh[7], fl[7], fs[7];
+        double dh[7], dl[7], ds[7];
+#pragma unroll
+        for (int k = 0; k < 7; k++) { fh[k] = fl[k] = fs[k] = 0.0f; dh[k] = dl[k] = ds[k] = 0.0; }
+        const float eps2f = float(d.eps2);
+        float ihx = d.hx[0][ip], ihy = d.hx[1][ip], ihz = d.hx[2][ip];
+        float ilx = d.lx[0][ip], ily = d.lx[1][ip], ilz = d.lx[2][ip];
+        float ivx = d.fv[0][ip], ivy = d.fv[1][ip], ivz = d.fv[2][ip];
+        const double ix = d.px[0][ip], iy = d.px[1][ip], iz = d.px[2][ip];
+        const double ivxd = d.pv[0][ip], ivyd = d.pv[1][ip], ivzd = d.pv[2][ip];
+        for (int tb = 0; tb < n; tb += kTJ) {
+            const int jn = min(kTJ, n - tb);
+            __syncthreads();
+            if (t < jn) {
+                const int q = off + tb + t;
+                if (MIXED) {
+                    tile.f.a[t] = make_float4(d.hx[0][q], d.hx[1][q], d.hx[2][q], d.fm[q]);
+                    tile.f.b[t] = make_float4(d.lx[0][q], d.lx[1][q], d.lx[2][q], d.fv[0][q]);
+                    tile.f.c[t] = make_float2(d.fv[1][q], d.fv[2][q]);
+                } else {
+                    tile.dd.a[t] = make_double4(d.px[0][q], d.px[1][q], d.px[2][q], d.m[q]);
+                    tile.dd.b[t] = make_double4(d.pv[0][q], d.pv[1][q], d.pv[2][q], 0.0);
+                }
+            }
+            __syncthreads();
+            int cntp = 0;
+            for (int jj = js; jj < jn; jj += njs) {
+                const bool self = (tb + jj) == i;
+                if (MIXED) {
+                    const float4 A = tile.f.a[jj], B = tile.f.b[jj];
+                    const float2 Cc = tile.f.c[jj];
+                    const float dx = (A.x - ihx) + (B.x - ilx);
+                    const float dy = (A.y - ihy) + (B.y - ily);
+                    const float dz = (A.z - ihz) + (B.z - ilz);
+                    const float s = __builtin_fmaf(dz, dz, __builtin_fmaf(dy, dy, __builtin_fmaf(dx, dx, eps2f)));
+                    float y = __builtin_amdgcn_rsqf(s);
+                    y = self ? 0.0f : y;
+                    const float y2 = y * y, mi = A.w * y, mi3 = mi * y2;
+                    fs[6] -= mi;
+                    fs[0] = __builtin_fmaf(mi3, dx, fs[0]);
+                    fs[1] = __builtin_fmaf(mi3, dy, fs[1]);
+                    fs[2] = __builtin_fmaf(mi3, dz, fs[2]);
+                    const float dvx = B.w - ivx, dvy = Cc.x - ivy, dvz = Cc.y - ivz;
+                    const float rv = __builtin_fmaf(dz, dvz, __builtin_fmaf(dy, dvy, dx * dvx));
+                    const float cc = (-3.0f * rv) * y2;
+                    fs[3] = __builtin_fmaf(mi3, __builtin_fmaf(cc, dx, dvx), fs[3]);
+                    fs[4] = __builtin_fmaf(mi3, __builtin_fmaf(cc, dy, dvy), fs[4]);
+                    fs[5] = __builtin_fmaf(mi3, __builtin_fmaf(cc, dz, dvz), fs[5]);
+                    if (++cntp == kFlushC) {
+                        cntp = 0;
+#pragma unroll
+                        for (int k = 0; k < 7; k++) { two_sum_f(fh[k], fl[k], fs[k]); fs[k] = 0.0f; }
+                    }
+                } else {
+                    const double4 A = tile.dd.a[jj], B = tile.dd.b[jj];
+                    const double dx = A.x - ix, dy = A.y - iy, dz = A.z - iz;
+                    const double s = __builtin_fma(dz, dz, __builtin_fma(dy, dy, __builtin_fma(dx, dx, d.eps2)));
+                    double y = __builtin_amdgcn_rsq(s);
+                    const double h = s * y;
+                    const double e = __builtin_fma(-h, y, 1.0);
+                    y = __builtin_fma(y * e, __builtin_fma(e, 0.375, 0.5), y);
+                    y = self ? 0.0 : y;
+                    const double y2 = y * y, mi = A.w * y, mi3 = mi * y2;
+                    ds[6] -= mi;
+                    ds[0] = __builtin_fma(mi3, dx, ds[0]);
+                    ds[1] = __builtin_fma(mi3, dy, ds[1]);
+                    ds[2] = __builtin_fma(mi3, dz, ds[2]);
+                    const double dvx = B.x - ivxd, dvy = B.y - ivyd, dvz = B.z - ivzd;
+                    const double rv = __builtin_fma(dz, dvz, __builtin_fma(dy, dvy, dx * dvx));
+                    const double cc = (-3.0 * rv) * y2;
+                    ds[3] = __builtin_fma(mi3, __builtin_fma(cc, dx, dvx), ds[3]);
+                    ds[4] = __builtin_fma(mi3, __builtin_fma(cc, dy, dvy), ds[4]);
+                    ds[5] = __builtin_fma(mi3, __builtin_fma(cc, dz, dvz), ds[5]);
+                }
+            }
+            if (MIXED) {
+#pragma unroll
+                for (int k = 0; k < 7; k++) { two_sum_f(fh[k], fl[k], fs[k]); fs[k] = 0.0f; }
+            } else {
+#pragma unroll
+                for (int k = 0; k < 7; k++) { two_sum_d(dh[k], dl[k], ds[k]); ds[k] = 0.0; }
+            }
+        }
+        // combine the njs lanes of each i
+        for (int o = 1; o < njs; o <<= 1) {
+#pragma unroll
+            for (int k = 0; k < 7; k++) {
+                if (MIXED) {
+                    const float oh = __shfl_xor(fh[k], o), ol = __shfl_xor(fl[k], o);
+                    two_sum_f(fh[k], fl[k], oh);
+                    fl[k] += ol;
+                } else {
+                    const double oh = __shfl_xor(dh[k], o), ol = __shfl_xor(dl[k], o);
+                    two_sum_d(dh[k], dl[k], oh);
+                    dl[k] += ol;
+                }
+            }
+        }
+        if (ok && js == 0) {
+            double r[7];
+#pragma unroll
+            for (int k = 0; k < 7; k++) r[k] = MIXED ? double(fh[k]) + double(fl[k]) : dh[k] + dl[k];
+            d.na[0][ip] = r[0]; d.na[1][ip] = r[1]; d.na[2][ip] = r[2];
+            d.nj[0][ip] = r[3]; d.nj[1][ip] = r[4]; d.nj[2][ip] = r[5];
+            d.npot[ip] = r[6];
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------
+// corrector + Aarseth step candidate (FP64), src/amuse_ph4/src/idata.cc:431-462
+// ---------------------------------------------------------------------------
+__global__ void __launch_bounds__(256) kc_correct(Dev d)
+{
+    const int p = blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= d.np) return;
+    const int c = d.pc[p];
+    if (!d.active[c]) return;
+    const double dt = d.dt[c];
+    const double dt2 = dt * dt;
+    double a2 = 0, j2 = 0, k2 = 0, l2 = 0;
+#pragma unroll
+    for (int k = 0; k < 3; k++) {
+        const double oa = d.a[k][p], oj = d.j[k][p];
+        const double ia = d.na[k][p], ij = d.nj[k][p];
+        const double alpha = -3 * (oa - ia) - dt * (2 * oj + ij);
+        const double beta = 2 * (oa - ia) + dt * (oj + ij);
+        d.x[k][p] = d.px[k][p] + (alpha / 12 + beta / 20) * dt2;
+        d.v[k][p] = d.pv[k][p] + (alpha / 3 + beta / 4) * dt;
+        d.a[k][p] = ia;
+        d.j[k][p] = ij;
+        a2 += ia * ia;
+        j2 += (dt * ij) * (dt * ij);
+        k2 += (2 * alpha) * (2 * alpha);
+        l2 += (6 * beta) * (6 * beta);
+    }
+    double cand = d.eta * dt * sqrt((sqrt(a2 * k2) + j2) / (sqrt(j2 * l2) + k2));
+    if (!(cand > 0.0)) cand = dt;  // degenerate sums: keep the step
+    atomicMin(&d.dtmin[c], dbits(cand));
+}
+
+// first step of each cluster: ph4's set_initial_timestep criterion
+// (jdata.cc:514-544), minimised over the cluster
+__global__ void __launch_bounds__(256) kc_initial_candidate(Dev d)
+{
+    const int p = blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= d.np) return;
+    const int c = d.pc[p];
+    double a2 = 0, j2 = 0;
+#pragma unroll
+    for (int k = 0; k < 3; k++) {
+        const double ia = d.na[k][p], ij = d.nj[k][p];
+        d.a[k][p] = ia;
+        d.j[k][p] = ij;
+        a2 += ia * ia;
+        j2 += ij * ij;
+    }
+    double first = (a2 == 0.0 || j2 == 0.0) ? d.dt_fac * d.eta : d.dt_fac * d.eta * sqrt(a2 / j2);
+    if (!(first > 0.0)) first = d.dt_fac * d.eta;
+    atomicMin(&d.dtmin[c], dbits(first));
+}
+
+__global__ void __launch_bounds__(256) kc_update(Dev d, int initial)
+{
+    const int c = blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= d.nc) return;
+    const double cand = __longlong_as_double(static_cast<long long>(d.dtmin[c]));
+    d.dtmin[c] = dbits(__builtin_inf());
+    if (initial) {
+        int e;
+        double s = cand / (2 * frexp(cand, &e));  // power of two <= cand
+        while (s > d.dt_limit) s /= 2;
+        d.dt[c] = s;
+        d.t[c] = 0.0;
+        d.active[c] = 1;
+        return;
+    }
+    if (!d.active[c]) return;
+    const double tn = d.t[c] + d.dt[c];
+    d.t[c] = tn;
+    d.nsteps[c] += 1;
+    double old2 = d.dt[c];
+    while (fmod(tn, old2) != 0) old2 /= 2;
+    double ns;
+    if (cand < old2) {
+        ns = old2 / 2;
+    } else {
+        const double t2 = 2 * old2;
+        ns = (cand >= t2 && fmod(tn, t2) == 0 && t2 <= d.dt_limit) ? t2 : old2;
+    }
+    d.dt[c] = ns;
+    if (tn >= d.t_end) {
+        d.active[c] = 0;
+        atomicSub(d.nactive, 1);
+    }
+}
+
+// Per-cluster energy from the FP64 potential of the last force pass at the
+// current state: E = sum m (v^2/2 + pot/2); one workgroup per cluster, fixed
+// reduction order.
+__global__ void __launch_bounds__(256) kc_energy(Dev d)
+{
+    __shared__ double red[256];
+    const int c = blockIdx.x;
+    const int off = d.off[c], n = d.cnt[c];
+    double e = 0.0;
+    for (int i = threadIdx.x; i < n; i += 256) {
+        const int p = off + i;
+        const double v2 = d.v[0][p] * d.v[0][p] + d.v[1][p] * d.v[1][p] + d.v[2][p] * d.v[2][p];
+        e += d.m[p] * (0.5 * v2 + 0.5 * d.npot[p]);
+    }
+    red[threadIdx.x] = e;
+    __syncthreads();
+    for (int s = 128; s > 0; s >>= 1) {
+        if (threadIdx.x < s) red[threadIdx.x] += red[threadIdx.x + s];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) d.energy[c] = red[0];
+}
+
+__global__ void kc_set_active_count(Dev d)
+{
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+        int n = 0;
+        for (int c = 0; c < d.nc; c++) n += d.active[c];
+        *d.nactive = n;
+    }
+}
+
+// ---------------------------------------------------------------------------
+// host side
+// ---------------------------------------------------------------------------
+struct Ensemble {
+    Dev d;
+    hipStream_t stream = nullptr;
+    void *base = nullptr;
+    int precision = 1;
+    std::vector<int> cnt, off;
+    int njs_class[kClasses];   // tuned lanes-per-i for each size class
+    int ib_rows = 0;
+    int persistent_wgs = 0;
+    hipGraphExec_t graph = nullptr;
+    int graph_steps = 0;
+    int graph_prec = -1;
+    double force_ms = 0.0;
+    long long launches = 0;
+    std::string err;
+};
+
+int size_class(int n)
+{
+    int k = 0, lim = 16;
+    while (n > lim && k < kClasses - 1) { lim *= 2; k++; }
+    return k;
+}
+
+#define CK(expr)                                                       \
+    do {                                                               \
+        hipError_t e_ = (expr);                                        \
+        if (e_ != hipSuccess) {                                        \
+            E->err = std::string(#expr) + ": " + hipGetErrorString(e_); \
+            return G6X_ERR_HIP;                                        \
+        }                                                              \
+    } while (0)
+
+// Work items: every (cluster, i-block) pair, i-block = 256/njs rows, sorted
+// by descending cost (rows x cluster size) so the queue ends with the cheap
+// ones.
+int build_items(Ensemble *E)
+{
+    // the previous item arrays may still be read by queued work
+    CK(hipStreamSynchronize(E->stream));
+    std::vector<int> ic, ii0, injs;
+    std::vector<double> cost;
+    for (int c = 0; c < E->d.nc; c++) {
+        const int n = E->cnt[c];
+        const int njs = E->njs_class[size_class(n)];
+        const int rows = kWG / njs;
+        for (int i0 = 0; i0 < n; i0 += rows) {
+            ic.push_back(c);
+            ii0.push_back(i0);
+            injs.push_back(njs);
+            cost.push_back(double(std::min(rows, n - i0)) * n / njs + 64.0 * n / kTJ);
+        }
+    }
+    std::vector<int> order(ic.size());
+    std::iota(order.begin(), order.end(), 0);
+    std::stable_sort(order.begin(), order.end(), [&](int a, int b) { return cost[a] > cost[b]; });
+    std::vector<int> a(order.size()), b(order.size()), cc(order.size());
+    for (size_t k = 0; k < order.size(); k++) {
+        a[k] = ic[order[k]];
+        b[k] = ii0[order[k]];
+        cc[k] = injs[order[k]];
+    }
+    const int n = int(order.size());
+    if (E->d.item_c) {
+        hipFree(E->d.item_c);
+        E->d.item_c = nullptr;
+    }
+    CK(hipMalloc(reinterpret_cast<void **>(&E->d.item_c), sizeof(int) * 3 * size_t(std::max(n, 1))));
+    E->d.item_i0 = E->d.item_c + n;
+    E->d.item_njs = E->d.item_c + 2 * n;
+    CK(hipMemcpy(E->d.item_c, a.data(), sizeof(int) * n, hipMemcpyHostToDevice));
+    CK(hipMemcpy(E->d.item_i0, b.data(), sizeof(int) * n, hipMemcpyHostToDevice));
+    CK(hipMemcpy(E->d.item_njs, cc.data(), sizeof(int) * n, hipMemcpyHostToDevice));
+    E->d.nitems = n;
+    if (E->graph) {
+        hipGraphExecDestroy(E->graph);
+        E->graph = nullptr;
+    }
+    return 0;
+}
+
+int launch_force(Ensemble *E, int prec)
+{
+    if (prec == G6X_MIXED)
+        hipLaunchKernelGGL(kc_force<true>, dim3(E->persistent_wgs), dim3(kWG), 0, E->stream, E->d);
+    else
+        hipLaunchKernelGGL(kc_force<false>, dim3(E->persistent_wgs), dim3(kWG), 0, E->stream, E->d);
+    return hipGetLastError() == hipSuccess ? 0 : G6X_ERR_HIP;
+}
+
+int enqueue_step(Ensemble *E, int prec)
+{
+    const int pb = (E->d.np + 255) / 256, cb = (E->d.nc + 255) / 256;
+    hipLaunchKernelGGL(kc_predict, dim3(pb), dim3(256), 0, E->stream, E->d, 0);
+    if (launch_force(E, prec)) return G6X_ERR_HIP;
+    hipLaunchKernelGGL(kc_correct, dim3(pb), dim3(256), 0, E->stream, E->d);
+    hipLaunchKernelGGL(kc_update, dim3(cb), dim3(256), 0, E->stream, E->d, 0);
+    return hipGetLastError() == hipSuccess ? 0 : G6X_ERR_HIP;
+}
+
+}  // namespace g6c
+
+using namespace g6c;
+
+// Creates an ensemble of nc clusters; cluster c holds counts[c] particles,
+// consecutive in m[], x[][3], v[][3].  precision: G6X_MIXED or G6X_FP64.
+G6C_API void *g6c_create(int nc, const int counts[], const double m[], const double xv[][3],
+                         const double vv[][3], double eps2, double eta, int precision)
+{
+    const double *x = &xv[0][0];
+    const double *v = &vv[0][0];
+    if (nc <= 0) return nullptr;
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev <= 0) return nullptr;
+    Ensemble *E = new Ensemble;
+    E->precision = precision;
+    E->cnt.assign(counts, counts + nc);
+    E->off.resize(nc);
+    long long np = 0;
+    for (int c = 0; c < nc; c++) {
+        if (counts[c] <= 0) { delete E; return nullptr; }
+        E->off[c] = int(np);
+        np += counts[c];
+    }
+    if (np > (1ll << 28)) { delete E; return nullptr; }
+    Dev &d = E->d;
+    d.nc = nc;
+    d.np = int(np);
+    d.eps2 = eps2;
+    d.eta = eta;
+    for (int k = 0; k < kClasses; k++) E->njs_class[k] = k <= 1 ? 16 : (k <= 3 ? 4 : 1);
+    if (hipStreamCreateWithFlags(&E->stream, hipStreamNonBlocking) != hipSuccess) { delete E; return nullptr; }
+    // one allocation: per-particle doubles (x v a j px pv na nj = 24, m, npot), floats (10),
+    // ints (pc), per-cluster doubles (t dt energy) + ull (dtmin) + ints (off cnt active nsteps)
+    const size_t P = size_t(np), Cn = size_t(nc);
+    auto al = [](size_t b) { return (b + 255) & ~size_t(255); };
+    size_t bytes = al(P * 8) * 26 + al(P * 4) * 11 + al(Cn * 8) * 4 + al(Cn * 4) * 4 + 256;
+    if (hipMalloc(&E->base, bytes) != hipSuccess) { delete E; return nullptr; }
+    char *q = static_cast<char *>(E->base);
+    auto D = [&](double *&p) { p = reinterpret_cast<double *>(q); q += al(P * 8); };
+    auto F = [&](float *&p) { p = reinterpret_cast<float *>(q); q += al(P * 4); };
+    for (int k = 0; k < 3; k++) { D(d.x[k]); D(d.v[k]); D(d.a[k]); D(d.j[k]); D(d.px[k]); D(d.pv[k]); D(d.na[k]); D(d.nj[k]); }
+    D(d.m); D(d.npot);
+    for (int k = 0; k < 3; k++) { F(d.hx[k]); F(d.lx[k]); F(d.fv[k]); }
+    F(d.fm);
+    d.pc = reinterpret_cast<int *>(q); q += al(P * 4);
+    d.t = reinterpret_cast<double *>(q); q += al(Cn * 8);
+    d.dt = reinterpret_cast<double *>(q); q += al(Cn * 8);
+    d.energy = reinterpret_cast<double *>(q); q += al(Cn * 8);
+    d.dtmin = reinterpret_cast<unsigned long long *>(q); q += al(Cn * 8);
+    d.off = reinterpret_cast<int *>(q); q += al(Cn * 4);
+    d.cnt = reinterpret_cast<int *>(q); q += al(Cn * 4);
+    d.active = reinterpret_cast<int *>(q); q += al(Cn * 4);
+    d.nsteps = reinterpret_cast<int *>(q); q += al(Cn * 4);
+    d.queue = reinterpret_cast<int *>(q);
+    d.nactive = d.queue + 1;
+
+    std::vector<double> h(P);
+    std::vector<float> hf(P);
+    std::vector<int> pc(P);
+    for (int c = 0; c < nc; c++)
+        for (int i = 0; i < counts[c]; i++) pc[E->off[c] + i] = c;
+    bool ok = true;
+    for (int k = 0; k < 3 && ok; k++) {
+        for (size_t p = 0; p < P; p++) h[p] = x[3 * p + k];
+        ok &= hipMemcpy(d.x[k], h.data(), P * 8, hipMemcpyHostToDevice) == hipSuccess;
+        for (size_t p = 0; p < P; p++) h[p] = v[3 * p + k];
+        ok &= hipMemcpy(d.v[k], h.data(), P * 8, hipMemcpyHostToDevice) == hipSuccess;
+    }
+    for (size_t p = 0; p < P; p++) hf[p] = float(m[p]);
+    ok &= hipMemcpy(d.m, m, P * 8, hipMemcpyHostToDevice) == hipSuccess;
+    ok &= hipMemcpy(d.fm, hf.data(), P * 4, hipMemcpyHostToDevice) == hipSuccess;
+    ok &= hipMemcpy(d.pc, pc.data(), P * 4, hipMemcpyHostToDevice) == hipSuccess;
+    ok &= hipMemcpy(d.off, E->off.data(), Cn * 4, hipMemcpyHostToDevice) == hipSuccess;
+    ok &= hipMemcpy(d.cnt, counts, Cn * 4, hipMemcpyHostToDevice) == hipSuccess;
+    std::vector<int> ones(nc, 1), zeros(nc, 0);
+    std::vector<unsigned long long> inf(nc, 0x7FF0000000000000ull);
+    ok &= hipMemcpy(d.active, ones.data(), Cn * 4, hipMemcpyHostToDevice) == hipSuccess;
+    ok &= hipMemcpy(d.nsteps, zeros.data(), Cn * 4, hipMemcpyHostToDevice) == hipSuccess;
+    ok &= hipMemcpy(d.dtmin, inf.data(), Cn * 8, hipMemcpyHostToDevice) == hipSuccess;
+    for (int k = 0; k < 3; k++) {
+        ok &= hipMemset(d.a[k], 0, P * 8) == hipSuccess;
+        ok &= hipMemset(d.j[k], 0, P * 8) == hipSuccess;
+    }
+    ok &= hipDeviceSynchronize() == hipSuccess;
+    int dev = 0, cus = 256;
+    hipGetDevice(&dev);
+    hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
+    E->persistent_wgs = cus * 4;
+    if (!ok || build_items(E)) {
+        hipFree(E->base);
+        delete E;
+        return nullptr;
+    }
+    return E;
+}
+
+G6C_API const char *g6c_last_error(void *h) { return static_cast<Ensemble *>(h)->err.c_str(); }
+
+// Forces at the current state (no prediction) in `precision`, then the
+// per-cluster energies from their FP64 potential when precision is FP64.
+static int force_now(Ensemble *E, int precision)
+{
+    const int pb = (E->d.np + 255) / 256;
+    hipLaunchKernelGGL(kc_predict, dim3(pb), dim3(256), 0, E->stream, E->d, 1);
+    if (launch_force(E, precision)) return G6X_ERR_HIP;
+    return 0;
+}
+
+// Initial forces and first steps; every cluster becomes active at t = 0.
+G6C_API int g6c_start(void *h, double t_end)
+{
+    Ensemble *E = static_cast<Ensemble *>(h);
+    E->d.t_end = t_end;
+    hipLaunchKernelGGL(kc_set_active_count, dim3(1), dim3(1), 0, E->stream, E->d);
+    // all clusters active for the initial pass
+    std::vector<int> ones(E->d.nc, 1);
+    CK(hipMemcpyAsync(E->d.active, ones.data(), sizeof(int) * E->d.nc, hipMemcpyHostToDevice, E->stream));
+    if (force_now(E, E->precision)) return G6X_ERR_HIP;
+    const int pb = (E->d.np + 255) / 256, cb = (E->d.nc + 255) / 256;
+    hipLaunchKernelGGL(kc_initial_candidate, dim3(pb), dim3(256), 0, E->stream, E->d);
+    hipLaunchKernelGGL(kc_update, dim3(cb), dim3(256), 0, E->stream, E->d, 1);
+    hipLaunchKernelGGL(kc_set_active_count, dim3(1), dim3(1), 0, E->stream, E->d);
+    CK(hipStreamSynchronize(E->stream));
+    return 0;
+}
+
+// Advances every cluster to t_end.  use_graph = 1 captures `steps_per_graph`
+// steps into one hipGraph and replays it; 0 launches the same kernels
+// eagerly.  Returns the number of step rounds executed.
+G6C_API long long g6c_run(void *h, int steps_per_graph, int use_graph, long long max_rounds)
+{
+    Ensemble *E = static_cast<Ensemble *>(h);
+    if (steps_per_graph < 1) steps_per_graph = 1;
+    if (use_graph && (!E->graph || E->graph_steps != steps_per_graph || E->graph_prec != E->precision)) {
+        if (E->graph) hipGraphExecDestroy(E->graph);
+        E->graph = nullptr;
+        hipGraph_t g;
+        CK(hipStreamBeginCapture(E->stream, hipStreamCaptureModeThreadLocal));
+        for (int s = 0; s < steps_per_graph; s++)
+            if (enqueue_step(E, E->precision)) {
+                hipStreamEndCapture(E->stream, &g);
+                return G6X_ERR_HIP;
+            }
+        CK(hipStreamEndCapture(E->stream, &g));
+        CK(hipGraphInstantiate(&E->graph, g, nullptr, nullptr, 0));
+        hipGraphDestroy(g);
+        E->graph_steps = steps_per_graph;
+        E->graph_prec = E->precision;
+    }
+    int *h_active = nullptr;
+    CK(hipHostMalloc(reinterpret_cast<void **>(&h_active), sizeof(int)));
+    long long rounds = 0;
+    hipEvent_t a, b;
+    hipEventCreate(&a);
+    hipEventCreate(&b);
+    hipEventRecord(a, E->stream);
+    for (;;) {
+        if (max_rounds > 0 && rounds >= max_rounds) break;
+        if (use_graph) {
+            CK(hipGraphLaunch(E->graph, E->stream));
+        } else {
+            for (int s = 0; s < steps_per_graph; s++)
+                if (enqueue_step(E, E->precision)) return G6X_ERR_HIP;
+        }
+        rounds += steps_per_graph;
+        CK(hipMemcpyAsync(h_active, E->d.nactive, sizeof(int), hipMemcpyDeviceToHost, E->stream));
+        CK(hipStreamSynchronize(E->stream));
+        if (*h_active <= 0) break;
+    }
+    hipEventRecord(b, E->stream);
+    hipEventSynchronize(b);
+    float ms = 0;
+    hipEventElapsedTime(&ms, a, b);
+    E->force_ms += ms;
+    hipEventDestroy(a);
+    hipEventDestroy(b);
+    hipHostFree(h_active);
+    return rounds;
+}
+
+// Per-cluster total energy at the current state, from an FP64 force pass.
+G6C_API int g6c_energy(void *h, double *energy)
+{
+    Ensemble *E = static_cast<Ensemble *>(h);
+    // force_now requires the clusters to be marked active; keep their flags
+    std::vector<int> saved(E->d.nc);
+    CK(hipMemcpyAsync(saved.data(), E->d.active, sizeof(int) * E->d.nc, hipMemcpyDeviceToHost, E->stream));
+    CK(hipStreamSynchronize(E->stream));
+    std::vector<int> ones(E->d.nc, 1);
+    CK(hipMemcpyAsync(E->d.active, ones.data(), sizeof(int) * E->d.nc, hipMemcpyHostToDevice, E->stream));
+    if (force_now(E, G6X_FP64)) return G6X_ERR_HIP;
+    hipLaunchKernelGGL(kc_energy, dim3(E->d.nc), dim3(256), 0, E->stream, E->d);
+    CK(hipMemcpyAsync(energy, E->d.energy, sizeof(double) * E->d.nc, hipMemcpyDeviceToHost, E->stream));
+    CK(hipMemcpyAsync(E->d.active, saved.data(), sizeof(int) * E->d.nc, hipMemcpyHostToDevice, E->stream));
+    CK(hipStreamSynchronize(E->stream));
+    return 0;
+}
+
+G6C_API int g6c_state(void *h, double xo[][3], double vo[][3], double t[], int nsteps[])
+{
+    double *x = &xo[0][0];
+    double *v = &vo[0][0];
+    Ensemble *E = static_cast<Ensemble *>(h);
+    const size_t P = size_t(E->d.np);
+    std::vector<double> buf(P);
+    for (int k = 0; k < 3; k++) {
+        CK(hipMemcpy(buf.data(), E->d.x[k], P * 8, hipMemcpyDeviceToHost));
+        for (size_t p = 0; p < P; p++) x[3 * p + k] = buf[p];
+        CK(hipMemcpy(buf.data(), E->d.v[k], P * 8, hipMemcpyDeviceToHost));
+        for (size_t p = 0; p < P; p++) v[3 * p + k] = buf[p];
+    }
+    if (t) CK(hipMemcpy(t, E->d.t, sizeof(double) * E->d.nc, hipMemcpyDeviceToHost));
+    if (nsteps) CK(hipMemcpy(nsteps, E->d.nsteps, sizeof(int) * E->d.nc, hipMemcpyDeviceToHost));
+    return 0;
+}
+
+// Forces of the current state (no prediction) for every particle, in the
+// given precision: acc, jerk [np][3], pot [np].
+G6C_API int g6c_forces(void *h, int precision, double acco[][3], double jerko[][3], double pot[])
+{
+    double *acc = &acco[0][0];
+    double *jerk = &jerko[0][0];
+    Ensemble *E = static_cast<Ensemble *>(h);
+    std::vector<int> saved(E->d.nc), ones(E->d.nc, 1);
+    CK(hipMemcpy(saved.data(), E->d.active, sizeof(int) * E->d.nc, hipMemcpyDeviceToHost));
+    CK(hipMemcpy(E->d.active, ones.data(), sizeof(int) * E->d.nc, hipMemcpyHostToDevice));
+    if (force_now(E, precision)) return G6X_ERR_HIP;
+    CK(hipStreamSynchronize(E->stream));
+    const size_t P = size_t(E->d.np);
+    std::vector<double> buf(P);
+    for (int k = 0; k < 3; k++) {
+        CK(hipMemcpy(buf.data(), E->d.na[k], P * 8, hipMemcpyDeviceToHost));
+        for (size_t p = 0; p < P; p++) acc[3 * p + k] = buf[p];
+        CK(hipMemcpy(buf.data(), E->d.nj[k], P * 8, hipMemcpyDeviceToHost));
+        for (size_t p = 0; p < P; p++) jerk[3 * p + k] = buf[p];
+    }
+    CK(hipMemcpy(pot, E->d.npot, P * 8, hipMemcpyDeviceToHost));
+    CK(hipMemcpy(E->d.active, saved.data(), sizeof(int) * E->d.nc, hipMemcpyHostToDevice));
+    return 0;
+}
+
+// Sets the lanes-per-i of each size class (<=16, <=32, ..., <=2048, more).
+G6C_API int g6c_set_tiles(void *h, const int *njs_per_class)
+{
+    Ensemble *E = static_cast<Ensemble *>(h);
+    for (int k = 0; k < kClasses; k++) {
+        const int v = njs_per_class[k];
+        if (v < 1 || v > 64 || (v & (v - 1))) return G6X_ERR_ARG;
+    }
+    std::copy(njs_per_class, njs_per_class + kClasses, E->njs_class);
+    return build_items(E);
+}
+
+G6C_API int g6c_get_tiles(void *h, int *njs_per_class)
+{
+    Ensemble *E = static_cast<Ensemble *>(h);
+    std::copy(E->njs_class, E->njs_class + kClasses, njs_per_class);
+    return kClasses;
+}
+
+// Autotuner: for each size class present, times the force pass restricted
+// to that class's clusters for njs in {1, 2, ..., 64} and keeps the fastest.
+G6C_API int g6c_autotune(void *h)
+{
+    Ensemble *E = static_cast<Ensemble *>(h);
+    std::vector<int> saved(E->d.nc);
+    CK(hipMemcpy(saved.data(), E->d.active, sizeof(int) * E->d.nc, hipMemcpyDeviceToHost));
+    hipEvent_t a, b;
+    hipEventCreate(&a);
+    hipEventCreate(&b);
+    int timed = 0;
+    for (int k = 0; k < kClasses; k++) {
+        std::vector<int> act(E->d.nc, 0);
+        bool any = false;
+        for (int c = 0; c < E->d.nc; c++)
+            if (size_class(E->cnt[c]) == k) { act[c] = 1; any = true; }
+        if (!any) continue;
+        CK(hipMemcpy(E->d.active, act.data(), sizeof(int) * E->d.nc, hipMemcpyHostToDevice));
+        float best = 1e30f;
+        int best_njs = E->njs_class[k];
+        for (int njs = 1; njs <= 64; njs *= 2) {
+            E->njs_class[k] = njs;
+            if (build_items(E)) return G6X_ERR_HIP;
+            hipLaunchKernelGGL(kc_predict, dim3((E->d.np + 255) / 256), dim3(256), 0, E->stream, E->d, 1);
+            launch_force(E, E->precision);
+            hipEventRecord(a, E->stream);
+            for (int r = 0; r < 3; r++) {
+                hipLaunchKernelGGL(kc_predict, dim3((E->d.np + 255) / 256), dim3(256), 0, E->stream, E->d, 1);
+                launch_force(E, E->precision);
+            }
+            hipEventRecord(b, E->stream);
+            CK(hipEventSynchronize(b));
+            float ms = 0;
+            hipEventElapsedTime(&ms, a, b);
+            timed++;
+            if (ms < best) { best = ms; best_njs = njs; }
+        }
+        E->njs_class[k] = best_njs;
+    }
+    hipEventDestroy(a);
+    hipEventDestroy(b);
+    if (build_items(E)) return G6X_ERR_HIP;
+    CK(hipMemcpy(E->d.active, saved.data(), sizeof(int) * E->d.nc, hipMemcpyHostToDevice));
+    return timed;
+}
+
+G6C_API int g6c_set_precision(void *h, int precision)
+{
+    Ensemble *E = static_cast<Ensemble *>(h);
+    if (precision != G6X_FP64 && precision != G6X_MIXED) return G6X_ERR_ARG;
+    E->precision = precision;
+    return 0;
+}
+
+G6C_API double g6c_elapsed_ms(void *h) { return static_cast<Ensemble *>(h)->force_ms; }
+
+G6C_API void g6c_free(void *h)
+{
+    Ensemble *E = static_cast<Ensemble *>(h);
+    if (!E) return;
+    hipStreamSynchronize(E->stream);
+    if (E->graph) hipGraphExecDestroy(E->graph);
+    if (E->d.item_c) hipFree(E->d.item_c);
+    if (E->base) hipFree(E->base);
+    hipStreamDestroy(E->stream);
+    delete E;
+}

@@ -68,6 +68,11 @@ struct Sim {
     std::vector<double> mass, time, step, pot;
     std::vector<double> pos, vel, acc, jerk;  // [n][3]
     double system_time = 0.0;
+    // ph4 does not reload the j-set after the initial force pass
+    // (idata::setup, idata.cc:66-83, has no update_gpu), so until its first
+    // step a particle is predicted with acc = jerk = 0; true reproduces
+    // that, false reloads every particle with its initial acc and jerk.
+    bool ph4_initial_load = true;
     long long block_steps = 0, particle_steps = 0, g6_calls = 0;
     // scratch
     std::vector<int> ilist, iid;
@@ -328,10 +333,20 @@ H4_API int h4_commit(void *h, int n, const int *ids, const double *m, const doub
         s.pot[j] = s.ipot[j];
     }
     initial_steps(s);
-    for (int j = 0; j < n; j++) {
-        rc = load_j(s, j);
-        if (rc < 0) return rc;
+    if (!s.ph4_initial_load) {
+        for (int j = 0; j < n; j++) {
+            rc = load_j(s, j);
+            if (rc < 0) return rc;
+        }
     }
+    return 0;
+}
+
+// true (default): the j-set keeps acc = jerk = 0 until each particle's first
+// step, as ph4 does; false: reload it with the initial acc and jerk.
+H4_API int h4_set_ph4_initial_load(void *h, int on)
+{
+    static_cast<Sim *>(h)->ph4_initial_load = on != 0;
     return 0;
 }
 

@@ -22,6 +22,7 @@ LIBPATH = os.path.join(HERE, "lib", "libg6.so")
 
 FP64 = 0
 MIXED = 1
+NGB_CAP = 256   # neighbour ids kept per i-particle
 
 _D = C.POINTER(C.c_double)
 _I = C.POINTER(C.c_int)
@@ -219,7 +220,8 @@ class G6:
         rc = self.lib.g6_get_neighbour_list(self.cluster, int(ipipe), int(maxlength),
                                             C.byref(n), _i(buf))
         self._check(rc, "g6_get_neighbour_list")
-        return rc, n.value, buf[:min(n.value, maxlength)].copy()
+        # ids copied: the full count, capped by the 256 kept and by maxlength
+        return rc, n.value, buf[:min(n.value, NGB_CAP, maxlength)].copy()
 
     # -- extensions (include/g6x.h) ---------------------------------------
     def set_precision(self, mode: int) -> None:

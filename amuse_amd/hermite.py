@@ -38,6 +38,7 @@ def _lib():
     L.h4_get_state.argtypes = [C.c_void_p, _D, _D]
     L.h4_get_steps.argtypes = [C.c_void_p, _D, _D]
     L.h4_set_precision.argtypes = [C.c_void_p, C.c_int]
+    L.h4_set_ph4_initial_load.argtypes = [C.c_void_p, C.c_int]
     L.h4_time.argtypes = [C.c_void_p]
     L.h4_time.restype = C.c_double
     L.h4_counters.argtypes = [C.c_void_p, _L, _L, _L]
@@ -48,12 +49,15 @@ def _lib():
 
 class Hermite4:
     def __init__(self, g6_library: str = G6_LIBPATH, npipes: int = 0, precision: int = 0,
-                 compat: bool = False, cluster: int = 0):
+                 compat: bool = False, cluster: int = 0, ph4_initial_load: bool = True):
         self.L = _lib()
         self.h = self.L.h4_new(g6_library.encode(), int(cluster), int(npipes), int(precision),
                                int(bool(compat)))
         if not self.h:
             raise RuntimeError(f"could not open GRAPE-6 library {g6_library}")
+        # ph4 leaves the j-set with acc = jerk = 0 after the initial force
+        # pass (idata.cc:66-83); False reloads it with the initial values
+        self.L.h4_set_ph4_initial_load(self.h, int(bool(ph4_initial_load)))
         self.n = 0
 
     def commit(self, m, x, v, eps2, eta=0.14, ids=None, dt_fac=0.0625, dt_limit=0.03125,

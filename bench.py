@@ -94,6 +94,89 @@ def load_traffic(kernel: str):
     return t.get(kernel)
 
 
+def copy_calibration() -> int:
+    """A plain device-to-device copy of exactly 1 GiB (torch's copy kernel),
+    for scaling the FETCH_SIZE / WRITE_SIZE counters to bytes."""
+    import torch
+    n = 1 << 28
+    a = torch.ones(n, dtype=torch.float32, device="cuda")
+    b = torch.empty_like(a)
+    torch.cuda.synchronize()
+    for _ in range(3):
+        b.copy_(a)
+    torch.cuda.synchronize()
+    print(json.dumps({"copy_bytes_read": 4 * n, "copy_bytes_written": 4 * n, "launches": 3}))
+    return 0
+
+
+def blocks_study(args) -> int:
+    """Force-only study: N = 65536 Plummer (equal masses, eps2 = 1e-4), every
+    j with its own time t_j = t_i - U(0,1) 2^-U{3..10}, acc and jerk from one
+    FP64 pass, i-sets = the first 2^k particles predicted to t_i, k = 0..16.
+    Per k: kernel time of both paths and their per-particle errors against the
+    80-bit oracle on (up to) 256 sampled i."""
+    import oracle
+    from amuse_amd import ic
+    from amuse_amd.g6 import G6
+    n, eps2 = 65536, 1e-4
+    rng = np.random.Generator(np.random.PCG64(65536))
+    m, x, v = ic.plummer(n, 1)
+    ids = np.arange(1, n + 1, dtype=np.int32)
+    g6 = G6(0)
+    g6.open()
+    z = np.zeros((n, 3))
+    g6.set_j_particles(np.arange(n), ids, np.zeros(n), 0.0, m, z, z, z, v, x)
+    g6.set_ti(0.0)
+    g6.calc_self(n, 0, n, eps2)
+    acc, jerk, _, _ = g6.fetch(n)
+    ti = 1.0
+    tj = ti - rng.uniform(0, 1, n) * 2.0 ** -rng.integers(3, 11, n).astype(float)
+    a2, j6 = acc / 2, jerk / 6
+    g6.set_j_particles(np.arange(n), ids, tj, 0.0, m, z, j6, a2, v, x)
+    g6.set_ti(ti)
+    xp, vp = oracle.predict(ti, tj, x, v, a2, j6)
+    rows = []
+    for k in range(0, 17):
+        ni = 1 << k
+        sel = np.arange(ni)
+        samp = sel if ni <= 256 else np.linspace(0, ni - 1, 256).astype(int)
+        ex = oracle.force_ld(ids[samp], xp[samp], vp[samp], eps2, ids, m, xp, vp)
+        row = {"k": k, "ni": ni}
+        for prec, name in ((0, "fp64"), (1, "mixed")):
+            g6.set_precision(prec)
+            g6.calc(n, ids[sel], xp[sel], vp[sel], eps2)     # warm-up
+            g6.timing(True)
+            reps = max(1, min(20, (1 << 22) // ni))
+            for _ in range(reps):
+                out = g6.calc(n, ids[sel], xp[sel], vp[sel], eps2)
+            tm = g6.get_timing()
+            g6.timing(False)
+            kms = tm["force_ms"] / tm["force_launches"]
+            pms = tm["predict_ms"]
+            errs = {}
+            for val, e, sm, q in ((out[0][samp], ex[0], ex[3], "acc"),
+                                  (out[1][samp], ex[1], ex[4], "jerk"),
+                                  (out[2][samp], ex[2], ex[5], "pot")):
+                d = np.abs(val - e).reshape(len(samp), -1)
+                d = np.sqrt((d * d).sum(1))
+                mag = np.abs(e).reshape(len(samp), -1)
+                mag = np.sqrt((mag * mag).sum(1))
+                rel, cond = d / mag, d / sm
+                errs[q] = {"rel_max": float(rel.max()), "rel_p99": float(np.quantile(rel, .99)),
+                           "cond_max": float(cond.max())}
+            row[name] = {"kernel_ms": kms, "gpairs_per_s": ni * n / kms / 1e6,
+                         "predict_ms_total": pms, "errors": errs}
+        rows.append(row)
+        log(f"  k={k}: fp64 {row['fp64']['gpairs_per_s']:.1f} mixed "
+            f"{row['mixed']['gpairs_per_s']:.1f} Gpairs/s; acc rel max fp64 "
+            f"{row['fp64']['errors']['acc']['rel_max']:.2e} mixed "
+            f"{row['mixed']['errors']['acc']['rel_max']:.2e}")
+    g6.close()
+    print(json.dumps({"workload": "i-blocks 2^k (k=0..16) vs N=65536 Plummer, per-j times, "
+                                  "eps2=1e-4", "rows": rows}))
+    return 0
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -103,8 +186,17 @@ def main() -> int:
     ap.add_argument("--eps2", type=float, default=1e-8)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-threads", type=int, default=0)
-    ap.add_argument("--cpu-per-thread", type=int, default=32)
+    ap.add_argument("--cpu-per-thread", type=int, default=256)
+    ap.add_argument("--copy-calib", action="store_true",
+                    help="only run a plain 1 GiB device copy (HBM counter calibration)")
+    ap.add_argument("--workload", default="full", choices=["full", "blocks"],
+                    help="full: full-N force pass at N=2^20 (default); blocks: i-blocks of "
+                         "2^k against N=65536 with per-j times, FP64 vs mixed errors")
     args = ap.parse_args()
+    if args.copy_calib:
+        return copy_calibration()
+    if args.workload == "blocks":
+        return blocks_study(args)
     if args.gpus != 1:
         print("bench.py measures one GPU: the force path is built and measured on a single "
               "MI355X (multi-GPU j-domain sharding is a design note only)", file=sys.stderr)

@@ -88,6 +88,56 @@ int g6x_autotune(int nj, int ni_max, double eps2);
 int g6x_save_tuning(const char *path);
 int g6x_load_tuning(const char *path);
 
+/* ---- Batched ensembles of independent clusters (g6_clusters.hip) --------
+ * Cluster c holds counts[c] consecutive particles of m[], x[][3], v[][3].
+ * Each cluster is integrated with 4th-order Hermite and one shared,
+ * power-of-two step (Aarseth criterion minimised over the cluster); all
+ * clusters advance together: one persistent force launch per step, the
+ * step's kernels captured in a hipGraph.  Returns NULL on failure. */
+void *g6c_create(int nc, const int counts[], const double m[],
+                 const double x[][3], const double v[][3], double eps2,
+                 double eta, int precision);
+/* initial forces and steps; clusters run until t_end */
+int g6c_start(void *ens, double t_end);
+/* step rounds (steps_per_graph per graph launch) until every cluster has
+ * reached t_end or max_rounds (0: no limit); returns the rounds run */
+long long g6c_run(void *ens, int steps_per_graph, int use_graph,
+                  long long max_rounds);
+/* per-cluster total energy from an FP64 force pass at the current state */
+int g6c_energy(void *ens, double energy[]);
+/* positions, velocities, per-cluster time and step count */
+int g6c_state(void *ens, double x[][3], double v[][3], double t[],
+              int nsteps[]);
+/* forces of the current state, unpredicted */
+int g6c_forces(void *ens, int precision, double acc[][3], double jerk[][3],
+               double pot[]);
+/* lanes per i-particle for each of 8 size classes (<=16, <=32, ...) */
+int g6c_set_tiles(void *ens, const int njs_per_class[8]);
+int g6c_get_tiles(void *ens, int njs_per_class[8]);
+/* times njs = 1..64 per size class present and keeps the fastest */
+int g6c_autotune(void *ens);
+int g6c_set_precision(void *ens, int precision);
+double g6c_elapsed_ms(void *ens);
+const char *g6c_last_error(void *ens);
+void g6c_free(void *ens);
+
+/* ---- All-pairs acc/jerk on caller-owned device arrays and its adjoint
+ * (g6_adjoint.hip).  x, v, acc, jerk, gx, gv, ga, gj are [n][3] row-major
+ * device arrays, m and gm [n]; stream is a hipStream_t (NULL: default).
+ * Forward: acc_i, jerk_i of the softened pair sum (self excluded).
+ * Adjoint: given ga = dL/dacc and gj = dL/djerk, writes gx = dL/dx,
+ * gv = dL/dv, gm = dL/dm.  Nothing is allocated. */
+int g6a_forces_f32(int n, const float *x, const float *v, const float *m,
+                   float eps2, float *acc, float *jerk, void *stream);
+int g6a_forces_f64(int n, const double *x, const double *v, const double *m,
+                   double eps2, double *acc, double *jerk, void *stream);
+int g6a_adjoint_f32(int n, const float *x, const float *v, const float *m,
+                    const float *ga, const float *gj, float eps2, float *gx,
+                    float *gv, float *gm, void *stream);
+int g6a_adjoint_f64(int n, const double *x, const double *v, const double *m,
+                    const double *ga, const double *gj, double eps2,
+                    double *gx, double *gv, double *gm, void *stream);
+
 #ifdef __cplusplus
 }
 #endif

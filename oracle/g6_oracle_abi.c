@@ -86,3 +86,22 @@ EXPORT int g6calc_lasthalf2_(int *cid, int *nj, int *ni, int index[], double xi[
     if (!U) return -2;
     return orc_unit_lasthalf2(U, index, &acc[0][0], &jerk[0][0], pot, nnb);
 }
+
+/* Remaining entry points with g6lib's no-op behaviour (g6lib.c:236-248,
+ * 443-481); reset empties the unit. */
+EXPORT int g6_set_tunit_(void *p) { (void)p; return 0; }
+EXPORT int g6_set_xunit_(void *p) { (void)p; return 0; }
+EXPORT int g6_initialize_jp_buffer_(int *c, int *b) { (void)c; (void)b; return 0; }
+EXPORT int g6_flush_jp_buffer_(int *c) { (void)c; return 0; }
+EXPORT int g6_reset_fofpga_(int *c) { (void)c; return 0; }
+EXPORT int g6_reset_(int *c)
+{
+    return g6_open_(c);
+}
+EXPORT int g6_read_neighbour_list_(int *c) { (void)c; return 0; }
+EXPORT int g6_get_neighbour_list_(int *c, int *ipipe, int *maxlength, int *n, int list[])
+{
+    (void)c; (void)ipipe; (void)maxlength; (void)list;
+    *n = 0;
+    return 0;
+}

@@ -1,0 +1,67 @@
+"""One run of a block-step Hermite integrator on a GRAPE-6 library, printed as
+JSON; run in its own process because each library holds one global unit.
+
+    python tests/ph4_run.py --integrator ph4|own --lib PATH --n N --t T [--seed S]
+
+ph4  the reference ph4 integrator (built unchanged by `make -C oracle ph4run`
+     into oracle/_ref/libph4run_<backend>.so, driven by oracle/ph4_harness.cc)
+own  amuse_amd/lib/libhermite4.so bound at run time to --lib
+"""
+import argparse
+import ctypes as C
+import json
+import os
+import sys
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+sys.path.insert(0, ROOT)
+
+from amuse_amd import ic  # noqa: E402
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--integrator", choices=["ph4", "own"], required=True)
+    ap.add_argument("--lib", required=True)
+    ap.add_argument("--n", type=int, default=256)
+    ap.add_argument("--t", type=float, default=0.125)
+    ap.add_argument("--seed", type=int, default=100)
+    ap.add_argument("--eps2", type=float, default=1e-4)
+    ap.add_argument("--eta", type=float, default=0.01)
+    ap.add_argument("--nout", type=int, default=4)
+    a = ap.parse_args()
+    m, x, v = ic.plummer(a.n, a.seed)
+    touts = a.t * np.arange(0, a.nout + 1) / a.nout   # includes t = 0
+    out = {"t": touts.tolist()}
+    if a.integrator == "ph4":
+        lib = C.CDLL(a.lib)
+        D, L = C.POINTER(C.c_double), C.POINTER(C.c_longlong)
+        nt = touts.size
+        k, p = np.zeros(nt), np.zeros(nt)
+        b, s = np.zeros(nt, np.int64), np.zeros(nt, np.int64)
+        rc = lib.ph4h_run(a.n, m.ctypes.data_as(D), x.ctypes.data_as(D), v.ctypes.data_as(D),
+                          C.c_double(a.eps2), C.c_double(a.eta), nt,
+                          touts.ctypes.data_as(D), k.ctypes.data_as(D), p.ctypes.data_as(D),
+                          b.ctypes.data_as(L), s.ctypes.data_as(L))
+        out.update(E=(k + p).tolist(), K=k.tolist(), blocks=b.tolist(), steps=s.tolist(),
+                   used_library=rc)
+    else:
+        from amuse_amd.hermite import Hermite4
+        h = Hermite4(a.lib)
+        h.commit(m, x, v, eps2=a.eps2, eta=a.eta)
+        E, K, B, S = [], [], [], []
+        for t in touts:
+            h.evolve(t)
+            kk, uu = h.energy()
+            c = h.counters()
+            E.append(kk + uu); K.append(kk)
+            B.append(c["block_steps"]); S.append(c["particle_steps"])
+        h.close()
+        out.update(E=E, K=K, blocks=B, steps=S)
+    print("RESULT " + json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,125 @@
+"""Batched ensembles on the GPU against the oracle (oracle/ensemble.py).
+
+* forces of every particle of every cluster: FP64 within 1e-15 sum|term| of
+  the 80-bit oracle, mixed within 2e-6 sum|term|;
+* FP64 integration of clusters in several size classes (so several lanes-
+  per-i shapes) follows the oracle: identical step counts, states within
+  1e-10 after 1/16 time unit;
+* a hipGraph replay equals eager launches bit for bit, replay after replay;
+* every tile shape gives the same FP64 forces to rounding;
+* mixed-precision energy error stays within a small factor of FP64's.
+"""
+import numpy as np
+import pytest
+
+import oracle
+from oracle import ensemble as oens
+from amuse_amd.ensemble import Ensemble, make_ensemble
+from conftest import print_case
+
+pytestmark = pytest.mark.gpu
+
+EPS2 = 1e-4
+ETA = 0.02
+
+
+def small_ensemble():
+    # sizes across classes <=16, <=32, <=128, <=512 and one above 2048
+    counts = np.array([12, 16, 31, 100, 257, 2100], dtype=np.int32)
+    from amuse_amd import ic
+    rng = np.random.Generator(np.random.PCG64(9))
+    ms, xs, vs = [], [], []
+    for n in counts:
+        mm = ic.kroupa(int(n), rng, 0.08, 20.0)
+        mm /= mm.sum()
+        _, x, v = ic.plummer_raw(int(n), rng)
+        x, v = ic.to_nbody_units(x, v)
+        x, v = ic.center(mm, x, v)
+        ms.append(mm); xs.append(x); vs.append(v)
+    return counts, np.concatenate(ms), np.concatenate(xs), np.concatenate(vs)
+
+
+def per_cluster(counts):
+    off = np.concatenate([[0], np.cumsum(counts)])
+    return [slice(off[c], off[c + 1]) for c in range(len(counts))]
+
+
+def test_ensemble_forces():
+    counts, m, x, v = small_ensemble()
+    ens = Ensemble(counts, m, x, v, eps2=EPS2, eta=ETA, precision=0)
+    a64, j64, p64 = ens.forces(0)
+    amx, jmx, pmx = ens.forces(1)
+    for sl in per_cluster(counts):
+        ids = np.arange(sl.stop - sl.start, dtype=np.int32)
+        ea, ej, ep, sa, sj, sp = oracle.force_ld(ids, x[sl], v[sl], EPS2, ids, m[sl], x[sl], v[sl])
+        for val, ex, s, tol in ((a64[sl], ea, sa, 1e-15), (j64[sl], ej, sj, 4e-15),
+                                (p64[sl], ep, sp, 1e-15)):
+            d = np.abs(val - ex).reshape(len(s), -1).max(1)
+            assert (d <= tol * s).all()
+        for val, ex, s in ((amx[sl], ea, sa), (jmx[sl], ej, sj), (pmx[sl], ep, sp)):
+            d = np.abs(val - ex).reshape(len(s), -1).max(1)
+            assert (d <= 2e-6 * s).all()
+    ens.close()
+
+
+def test_ensemble_fp64_integration_matches_oracle():
+    counts, m, x, v = small_ensemble()
+    counts, m, x, v = counts[:5], m[:counts[:5].sum()], x[:counts[:5].sum()], v[:counts[:5].sum()]
+    ens = Ensemble(counts, m, x, v, eps2=EPS2, eta=ETA, precision=0)
+    ens.start(0.0625)
+    ens.run(steps_per_graph=4, use_graph=True)
+    xg, vg, tg, ng = ens.state()
+    ens.close()
+    for c, sl in enumerate(per_cluster(counts)):
+        print_case(f"ensemble cluster {c} n={counts[c]}")
+        xo, vo, to, so = oens.integrate_cluster(m[sl], x[sl], v[sl], EPS2, ETA, 0.0625)
+        assert tg[c] == to == 0.0625
+        assert ng[c] == so, (c, ng[c], so)
+        assert np.abs(xg[sl] - xo).max() < 1e-10
+        assert np.abs(vg[sl] - vo).max() < 1e-9
+
+
+def test_graph_replay_equals_eager():
+    counts, m, x, v = small_ensemble()
+    res = []
+    for use_graph in (True, False):
+        ens = Ensemble(counts, m, x, v, eps2=EPS2, eta=ETA, precision=1)
+        ens.start(0.125)
+        rounds = ens.run(steps_per_graph=3, use_graph=use_graph)
+        res.append((ens.state(), rounds))
+        ens.close()
+    (sa, ra), (sb, rb) = res
+    assert ra == rb
+    for p, q in zip(sa, sb):
+        assert np.array_equal(p, q)
+
+
+def test_tile_shapes_agree():
+    counts, m, x, v = small_ensemble()
+    ens = Ensemble(counts, m, x, v, eps2=EPS2, eta=ETA, precision=0)
+    ref = ens.forces(0)
+    for njs in (1, 2, 4, 8, 16, 32, 64):
+        ens.set_tiles([njs] * 8)
+        got = ens.forces(0)
+        for g, r in zip(got, ref):
+            assert np.abs(g - r).max() <= 1e-13 * np.abs(r).max()
+    assert ens.autotune() > 0
+    tuned = ens.forces(0)
+    assert np.abs(tuned[0] - ref[0]).max() <= 1e-13 * np.abs(ref[0]).max()
+    ens.close()
+
+
+def test_mixed_energy_error_vs_fp64():
+    counts, m, x, v = make_ensemble(64, 5, nmin=16, nmax=256)
+    de = {}
+    for prec in (0, 1):
+        ens = Ensemble(counts, m, x, v, eps2=1e-4, eta=ETA, precision=prec)
+        e0 = ens.energy()
+        ens.start(0.25)
+        ens.run(steps_per_graph=8)
+        e1 = ens.energy()
+        de[prec] = np.abs((e1 - e0) / e0)
+        ens.close()
+    print("median |dE/E| fp64", np.median(de[0]), "mixed", np.median(de[1]))
+    assert np.median(de[0]) < 1e-7
+    assert np.median(de[1]) < 10 * np.median(de[0]) + 1e-8
