@@ -308,14 +308,10 @@ __device__ __forceinline__ void pair_f64(const ID &I, AccD &a, double xj, double
                                          int addr, double eps2)
 {
     const double dx = xj - I.x, dy = yj - I.y, dz = zj - I.z;
-    double r2, s;
-    if (NN) {
-        r2 = exact_r2(dx, dy, dz);
-        s = r2 + eps2;
-    } else {
-        r2 = 0.0;
-        s = __builtin_fma(dz, dz, __builtin_fma(dy, dy, __builtin_fma(dx, dx, eps2)));
-    }
+    // r^2 and s = r^2 + eps2 rounded as lib/g6lib/g6lib.c:149,156 does, in
+    // both variants, so forces do not depend on whether nn was asked for
+    const double r2 = exact_r2(dx, dy, dz);
+    const double s = r2 + eps2;
     const bool valid = idj != I.id;
     double y = rsqrt_f64(s);
     y = valid ? y : 0.0;
@@ -552,14 +548,9 @@ __device__ __forceinline__ void pair_mx(const IF &I, AccF &a, float4 ja, float4 
     const float dy = (ja.y - I.hy) + (jb.y - I.ly);
     const float dz = (ja.z - I.hz) + (jb.z - I.lz);
     const int idj = __float_as_int(jc.z);
-    float r2, s;
-    if (NN) {
-        r2 = __builtin_fmaf(dz, dz, __builtin_fmaf(dy, dy, dx * dx));
-        s = r2 + eps2;
-    } else {
-        r2 = 0.0f;
-        s = __builtin_fmaf(dz, dz, __builtin_fmaf(dy, dy, __builtin_fmaf(dx, dx, eps2)));
-    }
+    // same s in both variants (forces independent of the nn request)
+    const float r2 = __builtin_fmaf(dz, dz, __builtin_fmaf(dy, dy, dx * dx));
+    const float s = r2 + eps2;
     const bool valid = idj != I.id;
     float y = __builtin_amdgcn_rsqf(s);
     y = valid ? y : 0.0f;

@@ -177,6 +177,36 @@ def blocks_study(args) -> int:
     return 0
 
 
+def tile_sweep(args) -> int:
+    """Force-kernel time per tile shape for ni = 65536 against nj = N."""
+    from amuse_amd.g6 import G6
+    n, ni = args.n, min(65536, args.n)
+    m, pos, vel = make_system(n)
+    g6 = G6(0)
+    g6.open()
+    upload(g6, m, pos, vel)
+    rows = []
+    for prec in (0, 1):
+        g6.set_precision(prec)
+        for njs, ipl, smem in ((1, 2, 1), (1, 1, 1), (1, 2, 0), (1, 1, 0), (2, 2, 0), (4, 1, 0)):
+            g6.set_tiles(njs, ipl, 0, smem)
+            g6.calc_self(n, 0, ni, args.eps2)
+            g6.sync()
+            g6.timing(True)
+            for _ in range(3):
+                g6.calc_self(n, 0, ni, args.eps2)
+            tm = g6.get_timing()
+            g6.timing(False)
+            k = tm["force_ms"] / tm["force_launches"]
+            rows.append({"prec": prec, "njs": njs, "ipl": ipl, "smem": smem, "kernel_ms": k,
+                         "gpairs_per_s": ni * n / k / 1e6})
+            log(f"  {rows[-1]}")
+    g6.set_tiles(0, 0)
+    g6.close()
+    print(json.dumps({"workload": f"tile sweep ni={ni} nj={n}", "rows": rows}))
+    return 0
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -189,7 +219,7 @@ def main() -> int:
     ap.add_argument("--cpu-per-thread", type=int, default=256)
     ap.add_argument("--copy-calib", action="store_true",
                     help="only run a plain 1 GiB device copy (HBM counter calibration)")
-    ap.add_argument("--workload", default="full", choices=["full", "blocks"],
+    ap.add_argument("--workload", default="full", choices=["full", "blocks", "tiles"],
                     help="full: full-N force pass at N=2^20 (default); blocks: i-blocks of "
                          "2^k against N=65536 with per-j times, FP64 vs mixed errors")
     args = ap.parse_args()
@@ -197,6 +227,8 @@ def main() -> int:
         return copy_calibration()
     if args.workload == "blocks":
         return blocks_study(args)
+    if args.workload == "tiles":
+        return tile_sweep(args)
     if args.gpus != 1:
         print("bench.py measures one GPU: the force path is built and measured on a single "
               "MI355X (multi-GPU j-domain sharding is a design note only)", file=sys.stderr)

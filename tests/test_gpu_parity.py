@@ -45,11 +45,11 @@ def errors(val, exact, sums):
     return rel, cond, mag / np.where(sums > 0, sums, 1)
 
 
-def assert_plain_parity(acc, jerk, pot, exact):
+def assert_plain_parity(acc, jerk, pot, exact, cond_scale=1.0):
     ea, ej, ep, sa, sj, sp = exact
     for val, ex, s, name in ((acc, ea, sa, "acc"), (jerk, ej, sj, "jerk"), (pot, ep, sp, "pot")):
         rel, cond, ratio = errors(val, ex, s)
-        assert cond.max() <= COND[name], (name, cond.max())
+        assert cond.max() <= cond_scale * COND[name], (name, cond.max())
         ok = ratio >= 1e-3
         if ok.any():
             assert rel[ok].max() <= REL, (name, rel[ok].max())
@@ -78,12 +78,21 @@ def test_golden_reference_vectors(g6):
         exact = oracle.force_ld(d["id_i"], d["xi"], d["vi"], float(d["eps2"]), ids,
                                 d["m"][:nj], xp, vp)
         assert_plain_parity(acc, jerk, pot, exact)
-        # the reference's own doubles are within the same bars of the exact sum
-        assert_plain_parity(d["acc"], d["jerk"], d["pot"], exact)
-        # lasthalf without nn gives the same forces
+        # the reference's own doubles (plain sequential sum of pow() terms)
+        # are within 10x the same bars of the exact sum
+        assert_plain_parity(d["acc"], d["jerk"], d["pot"], exact, cond_scale=10.0)
+        # lasthalf without nn gives bit-identical forces
         acc2, jerk2, pot2, _ = g6.calc(nj, d["id_i"], d["xi"], d["vi"], float(d["eps2"]), nn=False)
         np.testing.assert_array_equal(acc2, acc)
+        np.testing.assert_array_equal(jerk2, jerk)
         np.testing.assert_array_equal(pot2, pot)
+        for prec in (1,):
+            g6.set_precision(prec)
+            r1 = g6.calc(nj, d["id_i"], d["xi"], d["vi"], float(d["eps2"]), nn=True)
+            r2 = g6.calc(nj, d["id_i"], d["xi"], d["vi"], float(d["eps2"]), nn=False)
+            for p_, q_ in zip(r1[:3], r2[:3]):
+                np.testing.assert_array_equal(p_, q_)
+            g6.set_precision(0)
 
 
 def plummer_state(n, seed, dt_nonzero=True):
