@@ -46,15 +46,15 @@ def _run(cmd: list[str]) -> None:
         raise RuntimeError("build failed: " + " ".join(cmd))
 
 
-def build_lib(name: str, isa: bool = False) -> str:
-    os.makedirs(LIBDIR, exist_ok=True)
+def build_lib(name: str, isa: bool = False, defines=(), outdir: str = LIBDIR) -> str:
+    os.makedirs(outdir, exist_ok=True)
     srcs = [os.path.join(CSRC, s) for s in LIBS[name]]
-    out = os.path.join(LIBDIR, name)
+    out = os.path.join(outdir, name)
     tmp = out + ".tmp"
     cmd = [hipcc(), "-O3", "-std=c++17", f"--offload-arch={ARCH}", "-fPIC", "-shared",
            "-fvisibility=hidden", "-Wno-unused-command-line-argument",
-           *DEVICE_FLAGS, "-I", os.path.join(ROOT, "include"), *srcs,
-           "-Wl,-Bsymbolic", "-o", tmp]
+           *DEVICE_FLAGS, *[f"-D{d}" for d in defines], "-I", os.path.join(ROOT, "include"),
+           *srcs, "-Wl,-Bsymbolic", "-o", tmp]
     _run(cmd)
     os.replace(tmp, out)
     if isa:

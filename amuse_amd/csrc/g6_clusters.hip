@@ -1,11 +1,12 @@
 // Batched ensembles: thousands of independent small N-body clusters advanced
 // together on one GPU.
 //
-// Scheme per cluster (restating the shared-timestep Hermite code
-// src/amuse_hermite/interface.cc:266-281 predictor, :536-616 step, with the
-// step size taken from ph4's Aarseth criterion and power-of-two
-// commensurability, src/amuse_ph4/src/idata.cc:431-512, minimised over the
-// cluster as hermite shares one step across its particles):
+// Scheme per cluster: the shared-timestep Hermite code
+// src/amuse_hermite/interface.cc — predictor :266-281, force with the
+// collision-time estimate :496-531, time-symmetric corrector :536-556, step
+// = dt_param x collision time :618-637 (dt_param = 0.03, :86) — with the step
+// rounded down to a power of two that divides the cluster time, so every
+// cluster lands exactly on a dyadic t_end:
 //
 //   predict   x_p = x + v dt + a dt^2/2 + j dt^3/6, v_p = v + a dt + j dt^2/2
 //             (FP64), stored again as double-single + FP32 for the force
@@ -13,11 +14,16 @@
 //             (cluster, i-block) items off a queue sorted by cost; NJS lanes
 //             share an i-particle (NJS from the tuned table of the cluster's
 //             size class); j-tiles of the cluster staged in LDS; mixed
-//             (double-single dx, FP32 r^-3, compensated sums) or FP64
-//   correct   ph4's Hermite corrector and Aarseth step (FP64), atomic minimum
-//             of the step over the cluster
-//   update    t += dt, next shared step (halve once, or double when
-//             commensurate), retire clusters that reached t_end
+//             (double-single dx, FP32 r^-3, compensated sums) or FP64; the
+//             inverse quartic collision time max(v^4/s^2, m_ij^2 r^2/s^4) is
+//             reduced per cluster with an atomic maximum
+//   correct   the time-symmetric corrector (FP64)
+//   update    t += dt, next step from the collision time, retire clusters
+//             that reached t_end
+//
+// The step criterion uses positions and velocities only: criteria built
+// from differences of successive forces (Aarseth's) turn FP32 force noise
+// into ever smaller steps, and would make the mixed path unusable here.
 //
 // The four kernels of a step contain no memset and no host synchronisation,
 // so a sequence of steps is captured once into a hipGraph and replayed.
@@ -45,7 +51,7 @@ struct Dev {
     int nc = 0, np = 0;
     int *off = nullptr, *cnt = nullptr, *pc = nullptr;  // cluster offsets/counts, particle->cluster
     double *t = nullptr, *dt = nullptr;
-    unsigned long long *dtmin = nullptr;
+    unsigned int *cmax = nullptr;  // per cluster: max over pairs of 1/collision-time^4 (float bits)
     int *active = nullptr, *nsteps = nullptr;
     double *x[3] = {}, *v[3] = {}, *a[3] = {}, *j[3] = {}, *m = nullptr;
     double *px[3] = {}, *pv[3] = {};
@@ -55,7 +61,7 @@ struct Dev {
     int *item_c = nullptr, *item_i0 = nullptr, *item_njs = nullptr;
     int nitems = 0;
     int *queue = nullptr, *nactive = nullptr;
-    double t_end = 1.0, eta = 0.02, eps2 = 1e-6, dt_limit = 0.03125, dt_fac = 0.0625;
+    double t_end = 1.0, dt_param = 0.03, eps2 = 1e-6, dt_limit = 0.03125, dt_floor = 0x1p-40;
 };
 
 __device__ __forceinline__ unsigned long long dbits(double x)
@@ -152,6 +158,8 @@ __global__ void __launch_bounds__(kWG) kc_force(Dev d)
         float ivx = d.fv[0][ip], ivy = d.fv[1][ip], ivz = d.fv[2][ip];
         const double ix = d.px[0][ip], iy = d.px[1][ip], iz = d.px[2][ip];
         const double ivxd = d.pv[0][ip], ivyd = d.pv[1][ip], ivzd = d.pv[2][ip];
+        const float imf = d.fm[ip];
+        float cq = 0.0f;  // max over j of 1 / (collision time)^4
         for (int tb = 0; tb < n; tb += kTJ) {
             const int jn = min(kTJ, n - tb);
             __syncthreads();
@@ -181,6 +189,15 @@ __global__ void __launch_bounds__(kWG) kc_force(Dev d)
                     y = self ? 0.0f : y;
                     const float y2 = y * y, mi = A.w * y, mi3 = mi * y2;
                     fs[6] -= mi;
+                    {
+                        // interface.cc:496-516: (s/v^2)^2 and s^4/(m_ij^2 r^2), inverted
+                        const float dvx = B.w - ivx, dvy = Cc.x - ivy, dvz = Cc.y - ivz;
+                        const float v2 = dvx * dvx + dvy * dvy + dvz * dvz;
+                        const float q1 = (v2 * y2) * (v2 * y2);
+                        const float u = (imf + A.w) * y2 * y2;
+                        const float q2 = u * u * (s - eps2f);
+                        cq = fmaxf(cq, fmaxf(q1, q2));
+                    }
                     fs[0] = __builtin_fmaf(mi3, dx, fs[0]);
                     fs[1] = __builtin_fmaf(mi3, dy, fs[1]);
                     fs[2] = __builtin_fmaf(mi3, dz, fs[2]);
@@ -206,6 +223,14 @@ __global__ void __launch_bounds__(kWG) kc_force(Dev d)
                     y = self ? 0.0 : y;
                     const double y2 = y * y, mi = A.w * y, mi3 = mi * y2;
                     ds[6] -= mi;
+                    {
+                        const double dvx = B.x - ivxd, dvy = B.y - ivyd, dvz = B.z - ivzd;
+                        const double v2 = dvx * dvx + dvy * dvy + dvz * dvz;
+                        const double q1 = (v2 * y2) * (v2 * y2);
+                        const double u = (double(imf) + A.w) * y2 * y2;
+                        const double q2 = u * u * (s - d.eps2);
+                        cq = fmaxf(cq, float(fmax(q1, q2)));
+                    }
                     ds[0] = __builtin_fma(mi3, dx, ds[0]);
                     ds[1] = __builtin_fma(mi3, dy, ds[1]);
                     ds[2] = __builtin_fma(mi3, dz, ds[2]);
@@ -240,6 +265,8 @@ __global__ void __launch_bounds__(kWG) kc_force(Dev d)
                 }
             }
         }
+        for (int o = 1; o < njs; o <<= 1) cq = fmaxf(cq, __shfl_xor(cq, o));
+        if (ok && js == 0) atomicMax(&d.cmax[c], __float_as_uint(cq));
         if (ok && js == 0) {
             double r[7];
 #pragma unroll
@@ -252,7 +279,7 @@ __global__ void __launch_bounds__(kWG) kc_force(Dev d)
 }
 
 // ---------------------------------------------------------------------------
-// corrector + Aarseth step candidate (FP64), src/amuse_ph4/src/idata.cc:431-462
+// time-symmetric corrector (FP64), src/amuse_hermite/interface.cc:536-556
 // ---------------------------------------------------------------------------
 __global__ void __launch_bounds__(256) kc_correct(Dev d)
 {
@@ -261,82 +288,66 @@ __global__ void __launch_bounds__(256) kc_correct(Dev d)
     const int c = d.pc[p];
     if (!d.active[c]) return;
     const double dt = d.dt[c];
-    const double dt2 = dt * dt;
-    double a2 = 0, j2 = 0, k2 = 0, l2 = 0;
 #pragma unroll
     for (int k = 0; k < 3; k++) {
-        const double oa = d.a[k][p], oj = d.j[k][p];
-        const double ia = d.na[k][p], ij = d.nj[k][p];
-        const double alpha = -3 * (oa - ia) - dt * (2 * oj + ij);
-        const double beta = 2 * (oa - ia) + dt * (oj + ij);
-        d.x[k][p] = d.px[k][p] + (alpha / 12 + beta / 20) * dt2;
-        d.v[k][p] = d.pv[k][p] + (alpha / 3 + beta / 4) * dt;
-        d.a[k][p] = ia;
-        d.j[k][p] = ij;
-        a2 += ia * ia;
-        j2 += (dt * ij) * (dt * ij);
-        k2 += (2 * alpha) * (2 * alpha);
-        l2 += (6 * beta) * (6 * beta);
+        const double oa = d.a[k][p], oj = d.j[k][p], ov = d.v[k][p], ox = d.x[k][p];
+        const double na = d.na[k][p], nj = d.nj[k][p];
+        const double v1 = ov + (oa + na) * dt / 2 + (oj - nj) * dt * dt / 12;
+        d.x[k][p] = ox + (ov + v1) * dt / 2 + (oa - na) * dt * dt / 12;
+        d.v[k][p] = v1;
+        d.a[k][p] = na;
+        d.j[k][p] = nj;
     }
-    double cand = d.eta * dt * sqrt((sqrt(a2 * k2) + j2) / (sqrt(j2 * l2) + k2));
-    if (!(cand > 0.0)) cand = dt;  // degenerate sums: keep the step
-    atomicMin(&d.dtmin[c], dbits(cand));
 }
 
-// first step of each cluster: ph4's set_initial_timestep criterion
-// (jdata.cc:514-544), minimised over the cluster
-__global__ void __launch_bounds__(256) kc_initial_candidate(Dev d)
+// first forces of each cluster become its current acc and jerk
+__global__ void __launch_bounds__(256) kc_take_forces(Dev d)
 {
     const int p = blockIdx.x * blockDim.x + threadIdx.x;
     if (p >= d.np) return;
-    const int c = d.pc[p];
-    double a2 = 0, j2 = 0;
 #pragma unroll
     for (int k = 0; k < 3; k++) {
-        const double ia = d.na[k][p], ij = d.nj[k][p];
-        d.a[k][p] = ia;
-        d.j[k][p] = ij;
-        a2 += ia * ia;
-        j2 += ij * ij;
+        d.a[k][p] = d.na[k][p];
+        d.j[k][p] = d.nj[k][p];
     }
-    double first = (a2 == 0.0 || j2 == 0.0) ? d.dt_fac * d.eta : d.dt_fac * d.eta * sqrt(a2 / j2);
-    if (!(first > 0.0)) first = d.dt_fac * d.eta;
-    atomicMin(&d.dtmin[c], dbits(first));
+}
+
+// Step from the collision time: dt_param x (1/cmax)^(1/4) (interface.cc:
+// 618-637), rounded down to a power of two dividing t and not above
+// dt_limit; never below dt_floor, so the cluster time always advances.
+__device__ __forceinline__ double next_step(const Dev &d, double t, unsigned int cbits)
+{
+    const float cm = __uint_as_float(cbits);
+    double cand = cm > 0.0f ? d.dt_param * pow(double(cm), -0.25) : d.dt_limit;
+    if (!(cand > 0.0)) cand = d.dt_floor;
+    double s = d.dt_limit;
+    for (int it = 0; it < 80 && s > cand && s > d.dt_floor; it++) s *= 0.5;
+    for (int it = 0; it < 80 && fmod(t, s) != 0.0 && s > d.dt_floor; it++) s *= 0.5;
+    return s < d.dt_floor ? d.dt_floor : s;
 }
 
 __global__ void __launch_bounds__(256) kc_update(Dev d, int initial)
 {
     const int c = blockIdx.x * blockDim.x + threadIdx.x;
     if (c >= d.nc) return;
-    const double cand = __longlong_as_double(static_cast<long long>(d.dtmin[c]));
-    d.dtmin[c] = dbits(__builtin_inf());
+    const unsigned int cb = d.cmax[c];
+    d.cmax[c] = 0u;
     if (initial) {
-        int e;
-        double s = cand / (2 * frexp(cand, &e));  // power of two <= cand
-        while (s > d.dt_limit) s /= 2;
-        d.dt[c] = s;
         d.t[c] = 0.0;
         d.active[c] = 1;
+        d.dt[c] = next_step(d, 0.0, cb);
         return;
     }
     if (!d.active[c]) return;
     const double tn = d.t[c] + d.dt[c];
     d.t[c] = tn;
     d.nsteps[c] += 1;
-    double old2 = d.dt[c];
-    while (fmod(tn, old2) != 0) old2 /= 2;
-    double ns;
-    if (cand < old2) {
-        ns = old2 / 2;
-    } else {
-        const double t2 = 2 * old2;
-        ns = (cand >= t2 && fmod(tn, t2) == 0 && t2 <= d.dt_limit) ? t2 : old2;
-    }
-    d.dt[c] = ns;
     if (tn >= d.t_end) {
         d.active[c] = 0;
         atomicSub(d.nactive, 1);
+        return;
     }
+    d.dt[c] = next_step(d, tn, cb);
 }
 
 // Per-cluster energy from the FP64 potential of the last force pass at the
@@ -481,7 +492,7 @@ using namespace g6c;
 // Creates an ensemble of nc clusters; cluster c holds counts[c] particles,
 // consecutive in m[], x[][3], v[][3].  precision: G6X_MIXED or G6X_FP64.
 G6C_API void *g6c_create(int nc, const int counts[], const double m[], const double xv[][3],
-                         const double vv[][3], double eps2, double eta, int precision)
+                         const double vv[][3], double eps2, double dt_param, int precision)
 {
     const double *x = &xv[0][0];
     const double *v = &vv[0][0];
@@ -503,11 +514,11 @@ G6C_API void *g6c_create(int nc, const int counts[], const double m[], const dou
     d.nc = nc;
     d.np = int(np);
     d.eps2 = eps2;
-    d.eta = eta;
+    d.dt_param = dt_param > 0.0 ? dt_param : 0.03;
     for (int k = 0; k < kClasses; k++) E->njs_class[k] = k <= 1 ? 16 : (k <= 3 ? 4 : 1);
     if (hipStreamCreateWithFlags(&E->stream, hipStreamNonBlocking) != hipSuccess) { delete E; return nullptr; }
     // one allocation: per-particle doubles (x v a j px pv na nj = 24, m, npot), floats (10),
-    // ints (pc), per-cluster doubles (t dt energy) + ull (dtmin) + ints (off cnt active nsteps)
+    // ints (pc), per-cluster doubles (t dt energy) + uint (cmax) + ints (off cnt active nsteps)
     const size_t P = size_t(np), Cn = size_t(nc);
     auto al = [](size_t b) { return (b + 255) & ~size_t(255); };
     size_t bytes = al(P * 8) * 26 + al(P * 4) * 11 + al(Cn * 8) * 4 + al(Cn * 4) * 4 + 256;
@@ -523,7 +534,7 @@ G6C_API void *g6c_create(int nc, const int counts[], const double m[], const dou
     d.t = reinterpret_cast<double *>(q); q += al(Cn * 8);
     d.dt = reinterpret_cast<double *>(q); q += al(Cn * 8);
     d.energy = reinterpret_cast<double *>(q); q += al(Cn * 8);
-    d.dtmin = reinterpret_cast<unsigned long long *>(q); q += al(Cn * 8);
+    d.cmax = reinterpret_cast<unsigned int *>(q); q += al(Cn * 8);
     d.off = reinterpret_cast<int *>(q); q += al(Cn * 4);
     d.cnt = reinterpret_cast<int *>(q); q += al(Cn * 4);
     d.active = reinterpret_cast<int *>(q); q += al(Cn * 4);
@@ -550,10 +561,10 @@ G6C_API void *g6c_create(int nc, const int counts[], const double m[], const dou
     ok &= hipMemcpy(d.off, E->off.data(), Cn * 4, hipMemcpyHostToDevice) == hipSuccess;
     ok &= hipMemcpy(d.cnt, counts, Cn * 4, hipMemcpyHostToDevice) == hipSuccess;
     std::vector<int> ones(nc, 1), zeros(nc, 0);
-    std::vector<unsigned long long> inf(nc, 0x7FF0000000000000ull);
+    std::vector<unsigned int> zero_u(nc, 0u);
     ok &= hipMemcpy(d.active, ones.data(), Cn * 4, hipMemcpyHostToDevice) == hipSuccess;
     ok &= hipMemcpy(d.nsteps, zeros.data(), Cn * 4, hipMemcpyHostToDevice) == hipSuccess;
-    ok &= hipMemcpy(d.dtmin, inf.data(), Cn * 8, hipMemcpyHostToDevice) == hipSuccess;
+    ok &= hipMemcpy(d.cmax, zero_u.data(), Cn * 4, hipMemcpyHostToDevice) == hipSuccess;
     for (int k = 0; k < 3; k++) {
         ok &= hipMemset(d.a[k], 0, P * 8) == hipSuccess;
         ok &= hipMemset(d.j[k], 0, P * 8) == hipSuccess;
@@ -594,7 +605,7 @@ G6C_API int g6c_start(void *h, double t_end)
     CK(hipMemcpyAsync(E->d.active, ones.data(), sizeof(int) * E->d.nc, hipMemcpyHostToDevice, E->stream));
     if (force_now(E, E->precision)) return G6X_ERR_HIP;
     const int pb = (E->d.np + 255) / 256, cb = (E->d.nc + 255) / 256;
-    hipLaunchKernelGGL(kc_initial_candidate, dim3(pb), dim3(256), 0, E->stream, E->d);
+    hipLaunchKernelGGL(kc_take_forces, dim3(pb), dim3(256), 0, E->stream, E->d);
     hipLaunchKernelGGL(kc_update, dim3(cb), dim3(256), 0, E->stream, E->d, 1);
     hipLaunchKernelGGL(kc_set_active_count, dim3(1), dim3(1), 0, E->stream, E->d);
     CK(hipStreamSynchronize(E->stream));

@@ -31,6 +31,14 @@ namespace g6k {
 
 static constexpr float kFarF = 1.0e18f;  // empty-slot position, mixed path
 
+// inner-loop unroll depths (build-time tunables)
+#ifndef G6_MX_UNROLL
+#define G6_MX_UNROLL 2
+#endif
+#ifndef G6_F64_UNROLL
+#define G6_F64_UNROLL 2
+#endif
+
 // ------------------------------------------------------------------------
 // Memory carving
 // ------------------------------------------------------------------------
@@ -421,7 +429,7 @@ __global__ void __launch_bounds__(kWG) k_force_f64(FParams p)
             tile.vzid[t] = make_double2(p.jvz[j], __longlong_as_double((long long)p.jid[j]));
         }
         __syncthreads();
-#pragma unroll 2
+#pragma unroll G6_F64_UNROLL
         for (int jj = js; jj < jn; jj += NJS) {
             const double2 a = tile.xy[jj], b = tile.zm[jj], c = tile.vxy[jj], d = tile.vzid[jj];
             const int idj = int(__double_as_longlong(d.y));
@@ -587,7 +595,10 @@ __device__ __forceinline__ void flush_mx(AccF &a, DSAcc &d)
 }
 
 // pairs per lane between two flushes of the FP32 sub-tile sums
-constexpr int kFlush = 64;
+#ifndef G6_MX_FLUSH
+#define G6_MX_FLUSH 64
+#endif
+constexpr int kFlush = G6_MX_FLUSH;
 
 template <int NJS, int IPL, bool NN>
 __global__ void __launch_bounds__(kWG) k_force_mx(FParams p)
@@ -633,7 +644,7 @@ __global__ void __launch_bounds__(kWG) k_force_mx(FParams p)
         __syncthreads();
         for (int jb = js; jb < jn; jb += NJS * kFlush) {
             const int je = min(jn, jb + NJS * kFlush);
-#pragma unroll 2
+#pragma unroll G6_MX_UNROLL
             for (int jj = jb; jj < je; jj += NJS) {
                 const float4 a = tile.a[jj], b = tile.b[jj], c = tile.c[jj];
 #pragma unroll

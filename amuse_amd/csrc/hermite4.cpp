@@ -187,7 +187,8 @@ void initial_steps(Sim &s)
         if (std::isnan(first)) first = s.dt_fac * s.eta;
         int e;
         first /= 2 * std::frexp(first, &e);
-        while (std::fmod(s.system_time, first) != 0) first /= 2;
+        for (int it = 0; it < 200 && first > 0 && std::fmod(s.system_time, first) != 0; it++)
+            first /= 2;
         while (first > s.dt_limit) first /= 2;
         s.step[j] = first;
     }
@@ -224,7 +225,9 @@ void correct(Sim &s, int q, int j, double tnext, const double *opos, const doubl
     double newstep = s.eta * dt * std::sqrt((std::sqrt(a2 * k2) + j2) / (std::sqrt(j2 * l2) + k2));
     int e;
     double old2 = s.step[j] / (2 * std::frexp(s.step[j], &e));
-    while (std::fmod(tnext, old2) != 0) old2 /= 2;
+    // bounded: tnext is a sum of powers of two no smaller than the steps
+    for (int it = 0; it < 200 && old2 > 0 && std::fmod(tnext, old2) != 0; it++) old2 /= 2;
+    if (!(newstep > 0)) newstep = old2;  // degenerate derivative sums
     if (newstep < old2) {
         newstep = old2 / 2;
     } else {

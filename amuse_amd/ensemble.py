@@ -1,11 +1,11 @@
 """Batched ensembles of independent clusters on one GPU (include/g6x.h g6c_*).
 
 Each cluster is integrated with the shared-step 4th-order Hermite scheme of
-src/amuse_hermite/interface.cc (predictor :266-281, step :558-616) with the
-step from ph4's Aarseth criterion (src/amuse_ph4/src/idata.cc:431-512)
-minimised over the cluster and kept a power of two commensurate with the
-cluster time; all clusters advance together (one persistent force launch
-per step, steps replayed from a hipGraph).
+src/amuse_hermite/interface.cc (predictor :266-281, collision-time estimate
+:496-531, corrector :536-556, step = dt_param x collision time :618-637)
+with the step rounded down to a power of two dividing the cluster time; all
+clusters advance together (one persistent force launch per step, steps
+replayed from a hipGraph).
 """
 from __future__ import annotations
 
@@ -43,7 +43,7 @@ def _bind(lib):
 
 
 class Ensemble:
-    def __init__(self, counts, m, x, v, eps2=1e-6, eta=0.02, precision=1):
+    def __init__(self, counts, m, x, v, eps2=1e-6, dt_param=0.03, precision=1):
         self.lib = _bind(load_library())
         self.counts = np.ascontiguousarray(counts, dtype=np.int32)
         self.m = np.ascontiguousarray(m, dtype=np.float64)
@@ -53,7 +53,8 @@ class Ensemble:
         v = np.ascontiguousarray(v, dtype=np.float64).reshape(n, 3)
         self.h = self.lib.g6c_create(self.counts.size, self.counts.ctypes.data_as(_I),
                                      self.m.ctypes.data_as(_D), x.ctypes.data_as(_D),
-                                     v.ctypes.data_as(_D), float(eps2), float(eta), int(precision))
+                                     v.ctypes.data_as(_D), float(eps2), float(dt_param),
+                                     int(precision))
         if not self.h:
             raise G6Error("g6c_create failed (no GPU or bad sizes)")
         self.n = n

@@ -207,6 +207,181 @@ def tile_sweep(args) -> int:
     return 0
 
 
+def evolve_blocks(m, pos, vel, eps2, eta, nblocks, precision):
+    """ph4-style run through the GRAPE-6 ABI: commit (full-N initial force)
+    then nblocks block steps with forces in `precision`; energies from FP64
+    passes.  Returns (dE/E, pairs of the block steps, seconds of the block
+    steps, block and particle step counts)."""
+    from amuse_amd.hermite import Hermite4
+    h = Hermite4(precision=precision)
+    h.commit(m, pos, vel, eps2=eps2, eta=eta)
+    h.set_precision(0)
+    e0 = sum(h.energy())
+    h.set_precision(precision)
+    c0 = h.counters()
+    t0 = time.perf_counter()
+    h.steps(nblocks)
+    dt = time.perf_counter() - t0
+    c1 = h.counters()
+    h.set_precision(0)
+    e1 = sum(h.energy())
+    h.close()
+    ps = c1["particle_steps"] - c0["particle_steps"]
+    return {"dE_over_E": (e1 - e0) / abs(e0), "E0": e0, "E1": e1,
+            "pairs": float(ps) * m.size, "seconds": dt,
+            "gpairs_per_s": float(ps) * m.size / dt / 1e9,
+            "block_steps": c1["block_steps"] - c0["block_steps"], "particle_steps": ps}
+
+
+def ph4_c0(args) -> int:
+    """Configuration c0: Plummer N = 1024, eps2 = 1e-4, eta = 0.01 to t = 10
+    with the block-step driver on libg6.so, FP64 and mixed forces, energy
+    error every 0.5 (measured with FP64 passes) beside the reference ph4
+    integrator's run on the CPU oracle (tests/golden/ph4_c0_energy.npz)."""
+    from amuse_amd import ic
+    from amuse_amd.hermite import Hermite4
+    gold = np.load(os.path.join(ROOT, "tests", "golden", "ph4_c0_energy.npz"))
+    m, x, v = ic.plummer(int(gold["n"]), int(gold["seed"]))
+    out = {"workload": "c0: Plummer N=1024, eps2=1e-4, eta=0.01, t=0..10",
+           "t": gold["t"].tolist(),
+           "reference_ph4_on_cpu_oracle": {
+               "dE_over_E": ((gold["E"] - gold["E"][0]) / abs(gold["E"][0])).tolist(),
+               "block_steps": gold["blocks"].tolist(), "particle_steps": gold["steps"].tolist()}}
+    for prec, name in ((0, "fp64"), (1, "mixed")):
+        h = Hermite4(precision=prec)
+        h.commit(m, x, v, eps2=float(gold["eps2"]), eta=float(gold["eta"]))
+        E, B, S = [], [], []
+        t0 = time.perf_counter()
+        for t in gold["t"]:
+            h.set_precision(prec)
+            h.evolve(float(t))
+            h.set_precision(0)
+            E.append(sum(h.energy()))
+            c = h.counters()
+            B.append(c["block_steps"]); S.append(c["particle_steps"])
+            log(f"  {name} t={t:.2f} dE/E={(E[-1] - E[0]) / abs(E[0]):.3e} blocks={B[-1]}")
+        wall = time.perf_counter() - t0
+        h.close()
+        E = np.array(E)
+        out[name] = {"dE_over_E": ((E - E[0]) / abs(E[0])).tolist(), "block_steps": B,
+                     "particle_steps": S, "wall_s": wall,
+                     "gpairs_per_s": float(S[-1]) * m.size / wall / 1e9}
+    print(json.dumps(out))
+    return 0
+
+
+def clusters_c2(args) -> int:
+    """Configuration c2: 4096 clusters, N log-uniform in [16, 2048], Kroupa
+    0.08-20 normalised to M = 1 per cluster, eps2 = 1e-6, all to t = 1 in
+    mixed precision with autotuned tiles and hipGraph-captured steps; the
+    same ensemble in FP64 for the energy comparison."""
+    from amuse_amd.ensemble import Ensemble, make_ensemble
+    nc = 4096 if args.clusters <= 0 else args.clusters
+    counts, m, x, v = make_ensemble(nc, 2048)
+    res = {"workload": f"c2: {nc} clusters, N log-uniform 16..2048, Kroupa, eps2=1e-6, t=1",
+           "particles": int(counts.sum()), "clusters": nc}
+    for prec, name in ((1, "mixed"), (0, "fp64")):
+        ens = Ensemble(counts, m, x, v, eps2=1e-6, dt_param=args.dt_param_c2, precision=prec)
+        tuned = ens.autotune()
+        tiles = ens.tiles()
+        e0 = ens.energy()
+        ens.start(1.0)
+        t0 = time.perf_counter()
+        rounds = ens.run(steps_per_graph=16, use_graph=True,
+                         max_rounds=args.max_rounds if args.max_rounds > 0 else 4000000)
+        wall = time.perf_counter() - t0
+        e1 = ens.energy()
+        _, _, tc, ns = ens.state()
+        ens.close()
+        pairs = float((ns.astype(np.float64) * counts.astype(np.float64) ** 2).sum())
+        de = np.abs((e1 - e0) / e0)
+        res[name] = {"wall_s": wall, "step_rounds": int(rounds),
+                     "cluster_steps_total": int(ns.sum()), "finished": int((tc >= 1.0).sum()),
+                     "gpairs_per_s": pairs / wall / 1e9, "tiles_njs_per_size_class": tiles,
+                     "autotune_timings": int(tuned),
+                     "dE_over_E_median": float(np.median(de)),
+                     "dE_over_E_p90": float(np.quantile(de, 0.9)),
+                     "dE_over_E_max": float(de.max())}
+        log(f"  {name}: {res[name]}")
+    res["mixed_over_fp64_speed"] = res["mixed"]["gpairs_per_s"] / res["fp64"]["gpairs_per_s"]
+    print(json.dumps(res))
+    return 0
+
+
+def adjoint_c3(args) -> int:
+    """Configuration c3: N = 16384 Plummer, equal masses, eps2 = 1e-4,
+    256 fixed Hermite steps of 1/256, checkpoint every 16; loss on the final
+    10/50/90 % Lagrangian radii (targets 0.9x initial); gradients w.r.t. x0,
+    v0, m in FP32, checked against the FP64 pipeline and central finite
+    differences."""
+    import torch
+    from amuse_amd import ic
+    from amuse_amd.diff import hermite_integrate, lagrangian_radii, radii_loss
+    n = args.n_c3
+    m, x, v = ic.plummer(n, 3)
+    dev = torch.device("cuda")
+    nsteps, dt, eps2, ck = args.steps_c3, 1.0 / 256, 1e-4, 16
+
+    def run(dtype, grads=True):
+        xx = torch.tensor(x, dtype=dtype, device=dev, requires_grad=grads)
+        vv = torch.tensor(v, dtype=dtype, device=dev, requires_grad=grads)
+        mm = torch.tensor(m, dtype=dtype, device=dev, requires_grad=grads)
+        with torch.no_grad():
+            targets = 0.9 * lagrangian_radii(xx, mm)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        xf, _ = hermite_integrate(xx, vv, mm, dt, nsteps, eps2, checkpoint_every=ck)
+        loss = radii_loss(xf, mm, targets)
+        torch.cuda.synchronize()
+        t1 = time.perf_counter()
+        if grads:
+            loss.backward()
+        torch.cuda.synchronize()
+        t2 = time.perf_counter()
+        g = (xx.grad, vv.grad, mm.grad) if grads else None
+        return float(loss), g, t1 - t0, t2 - t1
+
+    run(torch.float32)   # warm-up
+    L32, g32, tf32, tb32 = run(torch.float32)
+    L64, g64, tf64, tb64 = run(torch.float64)
+    pairs_fwd = float(n) * n * (nsteps + 1)
+    # the backward recomputes each checkpointed segment (forward pairs again)
+    # and runs one adjoint pass per force evaluation
+    rel = [float(torch.linalg.vector_norm(a.double() - b) / torch.linalg.vector_norm(b))
+           for a, b in zip(g32, g64)]
+    # central finite differences of the FP64 loss along two random directions
+    gen = torch.Generator(device=dev).manual_seed(11)
+    fd = []
+    for which in (0, 1):
+        d = torch.randn((n, 3), dtype=torch.float64, device=dev, generator=gen)
+        h = 1e-6
+        base = [torch.tensor(q, dtype=torch.float64, device=dev) for q in (x, v, m)]
+
+        def loss_at(sign):
+            b = [t.clone() for t in base]
+            b[which] = b[which] + sign * h * d
+            with torch.no_grad():
+                tg = 0.9 * lagrangian_radii(torch.tensor(x, dtype=torch.float64, device=dev),
+                                            torch.tensor(m, dtype=torch.float64, device=dev))
+                xf, _ = hermite_integrate(b[0], b[1], b[2], dt, nsteps, eps2, checkpoint_every=0)
+                return float(radii_loss(xf, b[2], tg))
+        num = (loss_at(1) - loss_at(-1)) / (2 * h)
+        an = float((g64[which] * d).sum())
+        fd.append({"wrt": ["x0", "v0"][which], "analytic": an, "finite_diff": num,
+                   "rel_err": abs(an - num) / max(abs(num), 1e-300)})
+    out = {"workload": f"c3: Plummer N={n}, {nsteps} Hermite steps dt=1/256, checkpoint {ck}, "
+                       "Lagrangian-radius loss",
+           "loss_fp32": L32, "loss_fp64": L64,
+           "fp32": {"forward_s": tf32, "backward_s": tb32,
+                    "forward_gpairs_per_s": pairs_fwd / tf32 / 1e9},
+           "fp64": {"forward_s": tf64, "backward_s": tb64,
+                    "forward_gpairs_per_s": pairs_fwd / tf64 / 1e9},
+           "grad_rel_err_fp32_vs_fp64": {"x0": rel[0], "v0": rel[1], "m": rel[2]},
+           "finite_difference_check_fp64": fd}
+    print(json.dumps(out))
+    return 0
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -214,12 +389,20 @@ def main() -> int:
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--n", type=int, default=1 << 20)
     ap.add_argument("--eps2", type=float, default=1e-8)
+    ap.add_argument("--eta", type=float, default=0.01, help="block steps: Aarseth eta")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-threads", type=int, default=0)
     ap.add_argument("--cpu-per-thread", type=int, default=256)
+    ap.add_argument("--clusters", type=int, default=0, help="c2: number of clusters")
+    ap.add_argument("--dt-param-c2", type=float, default=0.03,
+                    help="c2: hermite dt_param (step = dt_param x collision time)")
+    ap.add_argument("--max-rounds", type=int, default=0)
+    ap.add_argument("--n-c3", type=int, default=16384)
+    ap.add_argument("--steps-c3", type=int, default=256)
     ap.add_argument("--copy-calib", action="store_true",
                     help="only run a plain 1 GiB device copy (HBM counter calibration)")
-    ap.add_argument("--workload", default="full", choices=["full", "blocks", "tiles"],
+    ap.add_argument("--workload", default="full",
+                    choices=["full", "blocks", "tiles", "ph4", "clusters", "adjoint"],
                     help="full: full-N force pass at N=2^20 (default); blocks: i-blocks of "
                          "2^k against N=65536 with per-j times, FP64 vs mixed errors")
     args = ap.parse_args()
@@ -229,6 +412,12 @@ def main() -> int:
         return blocks_study(args)
     if args.workload == "tiles":
         return tile_sweep(args)
+    if args.workload == "ph4":
+        return ph4_c0(args)
+    if args.workload == "clusters":
+        return clusters_c2(args)
+    if args.workload == "adjoint":
+        return adjoint_c3(args)
     if args.gpus != 1:
         print("bench.py measures one GPU: the force path is built and measured on a single "
               "MI355X (multi-GPU j-domain sharding is a design note only)", file=sys.stderr)
@@ -283,6 +472,13 @@ def main() -> int:
                          f"{threads} independent units on {threads} threads, {sec:.1f} s",
                "max_rel_acc_vs_gpu_fp64": float(dc.max())}
 
+    g6.close()
+    # c4's 8 block steps after the full-N evaluation, both paths, energies
+    # from FP64 passes (the ph4-style driver over the same library)
+    log("bench: 8 block steps, FP64 then mixed forces")
+    evo = {"fp64": evolve_blocks(m, pos, vel, eps2, args.eta, 8, 0),
+           "mixed": evolve_blocks(m, pos, vel, eps2, args.eta, 8, 1)}
+
     traffic = load_traffic("mixed_full_n")
     out = {
         "metric": "G pair-interactions/s (acc+jerk+pot) at N=1M, mixed vs FP64 at stated "
@@ -317,10 +513,14 @@ def main() -> int:
                      "traffic": traffic, "kernel_ms": kmx,
                      "flop_per_pair": FLOP_PER_PAIR, "pairs_per_launch": pairs_per_launch_mx},
         "cpu_baseline": cpu,
+        "block_steps_8": evo,
+        "energy_error": {"fp64": evo["fp64"]["dE_over_E"], "mixed": evo["mixed"]["dE_over_E"],
+                         "note": "relative energy change over 8 ph4 block steps after the "
+                                 "full-N evaluation, forces in the named path, energies "
+                                 "from FP64 passes"},
     }
     if cpu:
         out["speedup_vs_cpu"] = {"mixed": gpmx / cpu["value"], "fp64": gp64 / cpu["value"]}
-    g6.close()
     print(json.dumps(out))
     return 0
 

@@ -90,13 +90,14 @@ int g6x_load_tuning(const char *path);
 
 /* ---- Batched ensembles of independent clusters (g6_clusters.hip) --------
  * Cluster c holds counts[c] consecutive particles of m[], x[][3], v[][3].
- * Each cluster is integrated with 4th-order Hermite and one shared,
- * power-of-two step (Aarseth criterion minimised over the cluster); all
- * clusters advance together: one persistent force launch per step, the
+ * Each cluster is integrated with the shared-step 4th-order Hermite scheme
+ * of the hermite code: step = dt_param x collision time (0.03 when
+ * dt_param <= 0), rounded down to a power of two dividing the cluster time;
+ * all clusters advance together: one persistent force launch per step, the
  * step's kernels captured in a hipGraph.  Returns NULL on failure. */
 void *g6c_create(int nc, const int counts[], const double m[],
                  const double x[][3], const double v[][3], double eps2,
-                 double eta, int precision);
+                 double dt_param, int precision);
 /* initial forces and steps; clusters run until t_end */
 int g6c_start(void *ens, double t_end);
 /* step rounds (steps_per_graph per graph launch) until every cluster has

@@ -1,12 +1,16 @@
 """CPU oracle of the batched-ensemble integrator — TEST INFRASTRUCTURE ONLY.
 
 Numpy restatement, cluster by cluster, of the scheme in
-amuse_amd/csrc/g6_clusters.hip:
-  predictor   src/amuse_hermite/interface.cc:266-281
-  corrector + Aarseth step   src/amuse_ph4/src/idata.cc:431-462
-  first step  src/amuse_ph4/src/jdata.cc:514-544 (minimised over the cluster)
-  step update (halve once, or double when commensurate)  idata.cc:464-510
-with forces from the oracle's 80-bit pair sum (g6_oracle.c orc_force_ld).
+amuse_amd/csrc/g6_clusters.hip, i.e. the shared-step Hermite code
+src/amuse_hermite/interface.cc:
+  predictor                 :266-281
+  collision-time estimate   :496-531 (min over pairs of (s/v^2)^2 and
+                            s^4/(m_ij^2 r^2), s = r^2 + eps2), evaluated at
+                            the predicted state of each step
+  time-symmetric corrector  :536-556
+  step = dt_param x collision time, dt_param = 0.03   :618-637, :86
+with the step rounded down to a power of two dividing the cluster time, and
+forces from the oracle's 80-bit pair sum (g6_oracle.c orc_force_ld).
 """
 from __future__ import annotations
 
@@ -24,45 +28,51 @@ def cluster_forces(m, x, v, eps2):
     return a, j, p
 
 
-def pow2_floor(x):
-    mant, e = math.frexp(x)
-    return x / (2 * mant)
+def collision_qmax(m, x, v, eps2):
+    """max over pairs of 1/(collision time)^4, rounded to float32 as the
+    device keeps it."""
+    r = x[None, :, :] - x[:, None, :]
+    w = v[None, :, :] - v[:, None, :]
+    r2 = (r * r).sum(-1)
+    s = r2 + eps2
+    v2 = (w * w).sum(-1)
+    q1 = (v2 / s) ** 2
+    mij = m[:, None] + m[None, :]
+    q2 = mij ** 2 * r2 / s ** 4
+    q = np.maximum(q1, q2)
+    np.fill_diagonal(q, 0.0)
+    return float(np.float32(q.max()))
 
 
-def integrate_cluster(m, x, v, eps2, eta, t_end, dt_fac=0.0625, dt_limit=0.03125):
+def next_step(t, qmax, dt_param=0.03, dt_limit=0.03125, dt_floor=2.0 ** -40):
+    cand = dt_param * qmax ** -0.25 if qmax > 0 else dt_limit
+    s = dt_limit
+    while s > cand and s > dt_floor:
+        s *= 0.5
+    while math.fmod(t, s) != 0.0 and s > dt_floor:
+        s *= 0.5
+    return max(s, dt_floor)
+
+
+def integrate_cluster(m, x, v, eps2, dt_param, t_end):
     x, v = x.copy(), v.copy()
     a, j, _ = cluster_forces(m, x, v, eps2)
-    a2 = (a * a).sum(1)
-    j2 = (j * j).sum(1)
-    first = np.where((a2 == 0) | (j2 == 0), dt_fac * eta, dt_fac * eta * np.sqrt(a2 / np.where(j2 == 0, 1, j2)))
-    dt = pow2_floor(float(first.min()))
-    while dt > dt_limit:
-        dt /= 2
+    dt = next_step(0.0, collision_qmax(m, x, v, eps2), dt_param)
     t, steps = 0.0, 0
-    while t < t_end:
+    while True:
         xp = x + v * dt + a * dt * dt / 2 + j * dt * dt * dt / 6
         vp = v + a * dt + j * dt * dt / 2
         a1, j1, _ = cluster_forces(m, xp, vp, eps2)
-        alpha = -3 * (a - a1) - dt * (2 * j + j1)
-        beta = 2 * (a - a1) + dt * (j + j1)
-        x = xp + (alpha / 12 + beta / 20) * dt * dt
-        v = vp + (alpha / 3 + beta / 4) * dt
-        A2 = (a1 * a1).sum(1)
-        J2 = ((dt * j1) ** 2).sum(1)
-        K2 = ((2 * alpha) ** 2).sum(1)
-        L2 = ((6 * beta) ** 2).sum(1)
-        cand = float((eta * dt * np.sqrt((np.sqrt(A2 * K2) + J2) / (np.sqrt(J2 * L2) + K2))).min())
+        q = collision_qmax(m, xp, vp, eps2)
+        v1 = v + (a + a1) * dt / 2 + (j - j1) * dt * dt / 12
+        x = x + (v + v1) * dt / 2 + (a - a1) * dt * dt / 12
+        v = v1
         a, j = a1, j1
         t += dt
         steps += 1
-        old2 = dt
-        while math.fmod(t, old2) != 0:
-            old2 /= 2
-        if cand < old2:
-            dt = old2 / 2
-        else:
-            t2 = 2 * old2
-            dt = t2 if (cand >= t2 and math.fmod(t, t2) == 0 and t2 <= dt_limit) else old2
+        if t >= t_end:
+            break
+        dt = next_step(t, q, dt_param)
     return x, v, t, steps
 
 
