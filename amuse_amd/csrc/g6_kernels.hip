@@ -1005,6 +1005,7 @@ bool force_cfg_supported(int njs, int ipl)
     if (ipl == 1) return njs == 1 || njs == 2 || njs == 4 || njs == 8 || njs == 16 ||
                          njs == 32 || njs == 64 || njs == 128 || njs == 256;
     if (ipl == 2) return njs == 1 || njs == 2 || njs == 4 || njs == 8;
+    if (ipl == 4) return njs == 1 || njs == 2;
     return false;
 }
 
@@ -1078,6 +1079,8 @@ hipError_t launch_force(const JStore &s, const IBlock &b, int ni, int nj, double
     case 22: launch_force_t<2, 2>(fp, prec, want_nn, grid, st); break;
     case 42: launch_force_t<4, 2>(fp, prec, want_nn, grid, st); break;
     case 82: launch_force_t<8, 2>(fp, prec, want_nn, grid, st); break;
+    case 14: launch_force_t<1, 4>(fp, prec, want_nn, grid, st); break;
+    case 24: launch_force_t<2, 4>(fp, prec, want_nn, grid, st); break;
     default: return hipErrorInvalidValue;
     }
     return hipGetLastError();

@@ -202,6 +202,8 @@ void initial_steps(Sim &s)
 }
 
 // idata::correct (idata.cc:431-512)
+constexpr double kStepFloor = 0x1p-40;
+
 void correct(Sim &s, int q, int j, double tnext, const double *opos, const double *ovel,
              const double *oacc, const double *ojerk)
 {
@@ -235,6 +237,10 @@ void correct(Sim &s, int q, int j, double tnext, const double *opos, const doubl
         if (newstep >= t2 && std::fmod(tnext, t2) == 0) newstep = t2;
         else newstep = old2;
     }
+    // Floor (never reached by FP64 forces): with force noise above the
+    // third difference of the forces the rule keeps halving; the floor keeps
+    // block times advancing and commensurate (all steps are powers of two).
+    if (newstep < kStepFloor) newstep = kStepFloor;
     s.time[j] = tnext;
     s.step[j] = newstep;
 }
@@ -355,19 +361,18 @@ H4_API int h4_set_ph4_initial_load(void *h, int on)
 
 // Block steps until the next block time would pass t_end (t_end itself is
 // reached when it is commensurate with the steps, e.g. any dyadic time).
+// Returns 0 then, or 1 when max_blocks (> 0) block steps ran first.
 H4_API int h4_evolve(void *h, double t_end, long long max_blocks)
 {
     Sim &s = *static_cast<Sim *>(h);
-    long long done = 0;
-    while (max_blocks <= 0 || done < max_blocks) {
+    for (long long done = 0;; done++) {
         double tnext = 1e300;
         for (int j = 0; j < s.n; j++) tnext = std::min(tnext, s.time[j] + s.step[j]);
-        if (tnext > t_end) break;
+        if (tnext > t_end) return 0;
+        if (max_blocks > 0 && done >= max_blocks) return 1;
         int rc = block_step(s);
         if (rc < 0) return rc;
-        done++;
     }
-    return 0;
 }
 
 H4_API int h4_steps(void *h, long long nblocks)

@@ -76,10 +76,12 @@ class Hermite4:
         self.n = n
         self.m = m
 
-    def evolve(self, t_end: float, max_blocks: int = 0) -> None:
+    def evolve(self, t_end: float, max_blocks: int = 0) -> bool:
+        """Block steps up to t_end; False if max_blocks (> 0) ran out first."""
         rc = self.L.h4_evolve(self.h, float(t_end), int(max_blocks))
         if rc < 0:
             raise RuntimeError(f"h4_evolve failed ({rc})")
+        return rc == 0
 
     def steps(self, nblocks: int) -> None:
         rc = self.L.h4_steps(self.h, int(nblocks))

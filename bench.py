@@ -235,9 +235,12 @@ def evolve_blocks(m, pos, vel, eps2, eta, nblocks, precision):
 
 def ph4_c0(args) -> int:
     """Configuration c0: Plummer N = 1024, eps2 = 1e-4, eta = 0.01 to t = 10
-    with the block-step driver on libg6.so, FP64 and mixed forces, energy
-    error every 0.5 (measured with FP64 passes) beside the reference ph4
-    integrator's run on the CPU oracle (tests/golden/ph4_c0_energy.npz)."""
+    with the block-step driver on libg6.so in FP64, energy error every 0.5
+    (FP64 passes) beside the reference ph4 integrator's run on the CPU oracle
+    (tests/golden/ph4_c0_energy.npz).  Mixed forces: at eta = 0.01 with a
+    block budget of 4x the FP64 run's (the step rule's third force difference
+    drops below FP32 rounding there), and FP64 and mixed at ph4's default
+    eta = 0.14 to t = 10."""
     from amuse_amd import ic
     from amuse_amd.hermite import Hermite4
     gold = np.load(os.path.join(ROOT, "tests", "golden", "ph4_c0_energy.npz"))
@@ -247,25 +250,38 @@ def ph4_c0(args) -> int:
            "reference_ph4_on_cpu_oracle": {
                "dE_over_E": ((gold["E"] - gold["E"][0]) / abs(gold["E"][0])).tolist(),
                "block_steps": gold["blocks"].tolist(), "particle_steps": gold["steps"].tolist()}}
-    for prec, name in ((0, "fp64"), (1, "mixed")):
+
+    def one(prec, eta, budget_per_out=None):
         h = Hermite4(precision=prec)
-        h.commit(m, x, v, eps2=float(gold["eps2"]), eta=float(gold["eta"]))
-        E, B, S = [], [], []
+        h.commit(m, x, v, eps2=float(gold["eps2"]), eta=eta)
+        E, B, S, T = [], [], [], []
         t0 = time.perf_counter()
-        for t in gold["t"]:
+        reached = True
+        for i, t in enumerate(gold["t"]):
             h.set_precision(prec)
-            h.evolve(float(t))
+            budget = 0 if budget_per_out is None else int(budget_per_out[i]) + 1
+            reached = h.evolve(float(t), budget)
             h.set_precision(0)
             E.append(sum(h.energy()))
             c = h.counters()
-            B.append(c["block_steps"]); S.append(c["particle_steps"])
-            log(f"  {name} t={t:.2f} dE/E={(E[-1] - E[0]) / abs(E[0]):.3e} blocks={B[-1]}")
+            B.append(c["block_steps"]); S.append(c["particle_steps"]); T.append(h.time)
+            log(f"  prec={prec} eta={eta} t={t:.2f} dE/E={(E[-1] - E[0]) / abs(E[0]):.3e} "
+                f"blocks={B[-1]} reached={reached}")
+            if not reached:
+                break
         wall = time.perf_counter() - t0
         h.close()
         E = np.array(E)
-        out[name] = {"dE_over_E": ((E - E[0]) / abs(E[0])).tolist(), "block_steps": B,
-                     "particle_steps": S, "wall_s": wall,
-                     "gpairs_per_s": float(S[-1]) * m.size / wall / 1e9}
+        return {"eta": eta, "dE_over_E": ((E - E[0]) / abs(E[0])).tolist(), "block_steps": B,
+                "particle_steps": S, "reached_all": reached, "time_reached": T[-1],
+                "wall_s": wall, "gpairs_per_s": float(S[-1]) * m.size / wall / 1e9}
+
+    eta0 = float(gold["eta"])
+    out["fp64"] = one(0, eta0)
+    budget = 4 * np.diff(np.concatenate([[0], gold["blocks"]]))
+    out["mixed"] = one(1, eta0, budget)
+    out["fp64_eta0.14"] = one(0, 0.14)
+    out["mixed_eta0.14"] = one(1, 0.14)
     print(json.dumps(out))
     return 0
 

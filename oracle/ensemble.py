@@ -36,9 +36,10 @@ def collision_qmax(m, x, v, eps2):
     r2 = (r * r).sum(-1)
     s = r2 + eps2
     v2 = (w * w).sum(-1)
-    q1 = (v2 / s) ** 2
     mij = m[:, None] + m[None, :]
-    q2 = mij ** 2 * r2 / s ** 4
+    with np.errstate(divide="ignore", invalid="ignore"):  # self pairs at eps2 = 0
+        q1 = (v2 / s) ** 2
+        q2 = mij ** 2 * r2 / s ** 4
     q = np.maximum(q1, q2)
     np.fill_diagonal(q, 0.0)
     return float(np.float32(q.max()))
