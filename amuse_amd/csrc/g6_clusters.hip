@@ -45,7 +45,8 @@ namespace g6c {
 constexpr int kWG = 256;
 constexpr int kTJ = 256;
 constexpr int kClasses = 8;  // size classes: <=16, <=32, ..., <=2048 (and above)
-constexpr int kFlushC = 64;
+constexpr int kFlushC = 64;  // pairs per lane between folds, FP32 sums
+constexpr int kFlushD = 32;  // the same for FP64 sums (clusters are not the hot FP64 path)
 
 struct Dev {
     int nc = 0, np = 0;
@@ -240,6 +241,11 @@ __global__ void __launch_bounds__(kWG) kc_force(Dev d)
                     ds[3] = __builtin_fma(mi3, __builtin_fma(cc, dx, dvx), ds[3]);
                     ds[4] = __builtin_fma(mi3, __builtin_fma(cc, dy, dvy), ds[4]);
                     ds[5] = __builtin_fma(mi3, __builtin_fma(cc, dz, dvz), ds[5]);
+                    if (++cntp == kFlushD) {
+                        cntp = 0;
+#pragma unroll
+                        for (int k = 0; k < 7; k++) { two_sum_d(dh[k], dl[k], ds[k]); ds[k] = 0.0; }
+                    }
                 }
             }
             if (MIXED) {

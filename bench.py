@@ -113,8 +113,9 @@ def blocks_study(args) -> int:
     """Force-only study: N = 65536 Plummer (equal masses, eps2 = 1e-4), every
     j with its own time t_j = t_i - U(0,1) 2^-U{3..10}, acc and jerk from one
     FP64 pass, i-sets = the first 2^k particles predicted to t_i, k = 0..16.
-    Per k: kernel time of both paths and their per-particle errors against the
-    80-bit oracle on (up to) 256 sampled i."""
+    Per k: kernel time of both paths, wall time of one ABI call (firsthalf +
+    lasthalf2 from host arrays, i.e. PCIe and synchronisation included), and
+    per-particle errors against the 80-bit oracle on (up to) 256 sampled i."""
     import oracle
     from amuse_amd import ic
     from amuse_amd.g6 import G6
@@ -147,8 +148,10 @@ def blocks_study(args) -> int:
             g6.calc(n, ids[sel], xp[sel], vp[sel], eps2)     # warm-up
             g6.timing(True)
             reps = max(1, min(20, (1 << 22) // ni))
+            w0 = time.perf_counter()
             for _ in range(reps):
                 out = g6.calc(n, ids[sel], xp[sel], vp[sel], eps2)
+            call_us = (time.perf_counter() - w0) / reps * 1e6
             tm = g6.get_timing()
             g6.timing(False)
             kms = tm["force_ms"] / tm["force_launches"]
@@ -165,6 +168,7 @@ def blocks_study(args) -> int:
                 errs[q] = {"rel_max": float(rel.max()), "rel_p99": float(np.quantile(rel, .99)),
                            "cond_max": float(cond.max())}
             row[name] = {"kernel_ms": kms, "gpairs_per_s": ni * n / kms / 1e6,
+                         "abi_call_us": call_us,
                          "predict_ms_total": pms, "errors": errs}
         rows.append(row)
         log(f"  k={k}: fp64 {row['fp64']['gpairs_per_s']:.1f} mixed "

@@ -9,8 +9,8 @@ GPU library follows the reference ph4 integrator's run on the CPU oracle
   counts are not compared exactly beyond the first output);
 * mixed forces at ph4's default timestep_parameter 0.14
   (src/amuse_ph4/interface.py:514-520): the run reaches t = 2 with step
-  counts within 10 % of an FP64 run at the same eta, and the energy error is
-  printed beside the FP64 one.  (At eta = 0.01 the corrector's third
+  counts within 10 % of an FP64 run at the same eta, and an energy error
+  within 2x the FP64 run's.  (At eta = 0.01 the corrector's third
   difference of the forces, ~a''' dt^3, falls to the size of FP32 force
   rounding and the steps shrink without bound; the mixed path is for the
   larger eta ph4 uses by default.)
@@ -71,4 +71,5 @@ def test_mixed_energy_error():
     print("|dE/E| fp64", de[0], "\n|dE/E| mixed", de[1], "\nblocks", res[0][2], res[1][2],
           flush=True)
     assert (np.abs(res[1][2] - res[0][2]) <= 0.1 * res[0][2]).all()
-    assert (de[1] < 1e-6).all()
+    # within 2x the FP64 run's energy error (measured: equal to ~5 %)
+    assert (de[1] <= 2 * de[0] + 1e-10).all()
