@@ -61,10 +61,11 @@ struct Unit {
     size_t d_stage_cap = 0;
     hipEvent_t ev_stage = nullptr;
 
-    double *h_in = nullptr;  // pinned [7][cap] x y z vx vy vz h2
-    int *h_id = nullptr;     // pinned [cap]
-    double *h_out = nullptr; // pinned [7][cap]
-    int *h_nn = nullptr;     // pinned [cap]
+    // pinned mirrors of the device i-block rows (same row pitch): [8][cap]
+    // x y z vx vy vz h2 id and [8][cap] acc3 jerk3 pot nn; id and nn are
+    // ints in the first half of their row
+    double *h_in = nullptr;
+    double *h_out = nullptr;
     int h_cap = 0;
     hipEvent_t ev_in = nullptr;
 
@@ -241,12 +242,10 @@ int grow_iblock(int need)
     IBlock nb;
     g6k::iblock_carve(nb, base, cap);
     U.ib = nb;
-    if (U.h_in) { hipHostFree(U.h_in); hipHostFree(U.h_id); hipHostFree(U.h_out); hipHostFree(U.h_nn); }
-    U.h_in = nullptr; U.h_id = nullptr; U.h_out = nullptr; U.h_nn = nullptr;
-    e = hipHostMalloc(reinterpret_cast<void **>(&U.h_in), sizeof(double) * 7 * size_t(cap));
-    if (e == hipSuccess) e = hipHostMalloc(reinterpret_cast<void **>(&U.h_id), sizeof(int) * size_t(cap));
-    if (e == hipSuccess) e = hipHostMalloc(reinterpret_cast<void **>(&U.h_out), sizeof(double) * 7 * size_t(cap));
-    if (e == hipSuccess) e = hipHostMalloc(reinterpret_cast<void **>(&U.h_nn), sizeof(int) * size_t(cap));
+    if (U.h_in) { hipHostFree(U.h_in); hipHostFree(U.h_out); }
+    U.h_in = nullptr; U.h_out = nullptr;
+    e = hipHostMalloc(reinterpret_cast<void **>(&U.h_in), sizeof(double) * 8 * size_t(cap));
+    if (e == hipSuccess) e = hipHostMalloc(reinterpret_cast<void **>(&U.h_out), sizeof(double) * 8 * size_t(cap));
     if (e != hipSuccess) return fail(G6X_ERR_NOMEM, "hipHostMalloc i/o", e);
     U.h_cap = cap;
     return 0;
@@ -503,20 +502,19 @@ int do_firsthalf(int nj, int ni, const int *index, const double (*xi)[3], const 
     if (ni > 0) {
         if (U.ev_in) HIPCHK(hipEventSynchronize(U.ev_in), "i event");
         const size_t C = size_t(U.ib.cap);
+        int *h_id = reinterpret_cast<int *>(U.h_in + 7 * C);
         for (int i = 0; i < ni; i++) {
             for (int k = 0; k < 3; k++) {
                 U.h_in[k * C + i] = xi[i][k];
                 U.h_in[(3 + k) * C + i] = vi[i][k];
             }
             U.h_in[6 * C + i] = h2 ? h2[i] : 0.0;
-            U.h_id[i] = index[i];
+            h_id[i] = index[i];
         }
-        for (int k = 0; k < 3; k++) {
-            HIPCHK(hipMemcpyAsync(U.ib.x[k], U.h_in + k * C, sizeof(double) * ni, hipMemcpyHostToDevice, U.stream), "i H2D");
-            HIPCHK(hipMemcpyAsync(U.ib.v[k], U.h_in + (3 + k) * C, sizeof(double) * ni, hipMemcpyHostToDevice, U.stream), "i H2D");
-        }
-        HIPCHK(hipMemcpyAsync(U.ib.h2, U.h_in + 6 * C, sizeof(double) * ni, hipMemcpyHostToDevice, U.stream), "i H2D");
-        HIPCHK(hipMemcpyAsync(U.ib.id, U.h_id, sizeof(int) * ni, hipMemcpyHostToDevice, U.stream), "i H2D");
+        // all eight rows in one pitched copy (the id row's second half is
+        // copied along, unused)
+        HIPCHK(hipMemcpy2DAsync(U.ib.x[0], sizeof(double) * C, U.h_in, sizeof(double) * C,
+                                sizeof(double) * ni, 8, hipMemcpyHostToDevice, U.stream), "i H2D");
         if (!U.ev_in) HIPCHK(hipEventCreateWithFlags(&U.ev_in, hipEventDisableTiming), "event");
         HIPCHK(hipEventRecord(U.ev_in, U.stream), "i record");
     }
@@ -536,10 +534,11 @@ int fetch_results(int ni, double (*acc)[3], double (*jerk)[3], double *pot, int 
 {
     if (ni <= 0) return 0;
     const size_t C = size_t(U.ib.cap);
-    // ib.out rows are [7][C]; one strided copy brings all seven columns
+    // result rows acc3 jerk3 pot (nn) follow each other at pitch C: one copy
     HIPCHK(hipMemcpy2DAsync(U.h_out, sizeof(double) * C, U.ib.out, sizeof(double) * C,
-                            sizeof(double) * ni, 7, hipMemcpyDeviceToHost, U.stream), "out D2H");
-    if (want_nn) HIPCHK(hipMemcpyAsync(U.h_nn, U.ib.nn, sizeof(int) * ni, hipMemcpyDeviceToHost, U.stream), "nn D2H");
+                            sizeof(double) * ni, want_nn ? 8 : 7, hipMemcpyDeviceToHost, U.stream),
+           "out D2H");
+    const int *h_nn = reinterpret_cast<const int *>(U.h_out + 7 * C);
     HIPCHK(hipStreamSynchronize(U.stream), "lasthalf sync");
     for (int i = 0; i < ni; i++) {
         for (int k = 0; k < 3; k++) {
@@ -547,7 +546,7 @@ int fetch_results(int ni, double (*acc)[3], double (*jerk)[3], double *pot, int 
             jerk[i][k] = U.h_out[(3 + k) * C + i];
         }
         pot[i] = U.h_out[6 * C + i];
-        if (want_nn) nn[i] = U.h_nn[i];
+        if (want_nn) nn[i] = h_nn[i];
     }
     return 0;
 }

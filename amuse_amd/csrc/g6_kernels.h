@@ -29,6 +29,7 @@ constexpr int kTJ = 256;        // j per LDS tile
 constexpr int kPartFields = 9;   // mixed: (hi,lo) float pairs of acc3 jerk3 pot, r2, slot
 constexpr int kPartFieldsD = 16; // plain: hi, lo of acc3 jerk3 pot, r2, slot
 constexpr double kFar = 1.0e150;  // position given to empty slots
+constexpr int kReduceWaveMin = 16;  // chunks from which the reduce runs one wave per i
 constexpr int kNgbCap = 256;      // neighbour ids kept per i (sapporo NGB_PP)
 
 struct JStore {

@@ -74,20 +74,22 @@ void jstore_carve(JStore &s, void *base, int cap)
 
 size_t iblock_bytes(int cap)
 {
-    size_t n = size_t(cap);
-    return align256(n * sizeof(double)) * (7 + 7) + align256(n * sizeof(int)) * 2;
+    return align256(size_t(cap) * sizeof(double)) * 16;
 }
 
+// Sixteen rows of cap doubles: x y z vx vy vz h2 id | acc3 jerk3 pot nn, so
+// that one pitched copy moves a whole i-block in and one moves the results
+// out (the id and nn rows hold ints in the first half of their row).
 void iblock_carve(IBlock &b, void *base, int cap)
 {
     char *p = static_cast<char *>(base);
-    size_t n = size_t(cap);
-    for (int k = 0; k < 3; k++) { b.x[k] = reinterpret_cast<double *>(p); p += align256(n * sizeof(double)); }
-    for (int k = 0; k < 3; k++) { b.v[k] = reinterpret_cast<double *>(p); p += align256(n * sizeof(double)); }
-    b.h2 = reinterpret_cast<double *>(p); p += align256(n * sizeof(double));
-    b.out = reinterpret_cast<double *>(p); p += align256(n * sizeof(double)) * 7;
-    b.id = reinterpret_cast<int *>(p); p += align256(n * sizeof(int));
-    b.nn = reinterpret_cast<int *>(p); p += align256(n * sizeof(int));
+    const size_t row = align256(size_t(cap) * sizeof(double));
+    for (int k = 0; k < 3; k++) b.x[k] = reinterpret_cast<double *>(p + k * row);
+    for (int k = 0; k < 3; k++) b.v[k] = reinterpret_cast<double *>(p + (3 + k) * row);
+    b.h2 = reinterpret_cast<double *>(p + 6 * row);
+    b.id = reinterpret_cast<int *>(p + 7 * row);
+    b.out = reinterpret_cast<double *>(p + 8 * row);
+    b.nn = reinterpret_cast<int *>(p + 15 * row);
     b.cap = cap;
     b.base = base;
 }
@@ -904,6 +906,91 @@ __global__ void __launch_bounds__(256) k_reduce_mx(const double *__restrict__ pa
     if (want_nn) nn[i] = addr >= 0 ? jid[addr] : -1;
 }
 
+// One wave per i for many chunks: lane L folds chunks L, L + 64, ... and a
+// fixed butterfly combines the lanes, so the result does not depend on
+// timing (the summation order differs from the one-lane form, which keeps
+// the latency chain of a long serial loop for small i-blocks).
+__global__ void __launch_bounds__(256) k_reduce_f64_w(const double *__restrict__ part, int nchunk,
+                                                      int ni, int S, const int *__restrict__ jid,
+                                                      double *__restrict__ out,
+                                                      int *__restrict__ nn, int want_nn)
+{
+    const int i = blockIdx.x * 4 + int(threadIdx.x >> 6);
+    const int lane = threadIdx.x & 63;
+    if (i >= ni) return;  // whole waves leave together
+    double h[7], l[7];
+#pragma unroll
+    for (int k = 0; k < 7; k++) { h[k] = 0.0; l[k] = 0.0; }
+    double best = __builtin_inf();
+    int addr = -1;
+    for (int ch = lane; ch < nchunk; ch += 64) {
+        const double *r = part + size_t(ch) * kPartFieldsD * S;
+#pragma unroll
+        for (int k = 0; k < 7; k++) {
+            two_sum_add_d(h[k], l[k], r[(2 * k) * S + i]);
+            l[k] += r[(2 * k + 1) * S + i];
+        }
+        if (want_nn) nn_merge(best, addr, r[14 * S + i], int(r[15 * S + i]));
+    }
+#pragma unroll
+    for (int off = 32; off >= 1; off >>= 1) {
+#pragma unroll
+        for (int k = 0; k < 7; k++) {
+            const double oh = __shfl_xor(h[k], off), ol = __shfl_xor(l[k], off);
+            two_sum_add_d(h[k], l[k], oh);
+            l[k] += ol;
+        }
+        const double ob = __shfl_xor(best, off);
+        const int oa = __shfl_xor(addr, off);
+        nn_merge(best, addr, ob, oa);
+    }
+    if (lane != 0) return;
+#pragma unroll
+    for (int k = 0; k < 7; k++) out[k * S + i] = h[k] + l[k];
+    if (want_nn) nn[i] = addr >= 0 ? jid[addr] : -1;
+}
+
+__global__ void __launch_bounds__(256) k_reduce_mx_w(const double *__restrict__ part, int nchunk,
+                                                     int ni, int S, const int *__restrict__ jid,
+                                                     double *__restrict__ out,
+                                                     int *__restrict__ nn, int want_nn)
+{
+    const int i = blockIdx.x * 4 + int(threadIdx.x >> 6);
+    const int lane = threadIdx.x & 63;
+    if (i >= ni) return;
+    float h[7], l[7];
+#pragma unroll
+    for (int k = 0; k < 7; k++) { h[k] = 0.0f; l[k] = 0.0f; }
+    float best = __builtin_inff();
+    int addr = -1;
+    for (int ch = lane; ch < nchunk; ch += 64) {
+        const float2 *r = reinterpret_cast<const float2 *>(part + size_t(ch) * kPartFields * S);
+#pragma unroll
+        for (int k = 0; k < 7; k++) {
+            const float2 v = r[k * S + i];
+            two_sum_add(h[k], l[k], v.x);
+            l[k] += v.y;
+        }
+        if (want_nn) nn_merge_f(best, addr, r[7 * S + i].x, __float_as_int(r[8 * S + i].x));
+    }
+#pragma unroll
+    for (int off = 32; off >= 1; off >>= 1) {
+#pragma unroll
+        for (int k = 0; k < 7; k++) {
+            const float oh = __shfl_xor(h[k], off), ol = __shfl_xor(l[k], off);
+            two_sum_add(h[k], l[k], oh);
+            l[k] += ol;
+        }
+        const float ob = __shfl_xor(best, off);
+        const int oa = __shfl_xor(addr, off);
+        nn_merge_f(best, addr, ob, oa);
+    }
+    if (lane != 0) return;
+#pragma unroll
+    for (int k = 0; k < 7; k++) out[k * S + i] = double(h[k]) + double(l[k]);
+    if (want_nn) nn[i] = addr >= 0 ? jid[addr] : -1;
+}
+
 // ------------------------------------------------------------------------
 // Neighbour query (lib/sapporo_light/dev_evaluate_gravity.cu:60-67 criterion
 // r^2 <= h2 with self excluded; empty slots excluded as in g6lib.c:138-140).
@@ -1090,6 +1177,16 @@ hipError_t launch_reduce(const JStore &s, const IBlock &b, int ni, int nchunk, i
                          bool want_nn, hipStream_t st)
 {
     if (ni <= 0 || nchunk <= 1) return hipSuccess;
+    if (nchunk >= kReduceWaveMin) {
+        // one wave per i, four i per workgroup
+        if (prec == kF64)
+            hipLaunchKernelGGL(k_reduce_f64_w, dim3(nblocks(ni, 4)), dim3(256), 0, st, b.part,
+                               nchunk, ni, b.cap, s.pid, b.out, b.nn, int(want_nn));
+        else
+            hipLaunchKernelGGL(k_reduce_mx_w, dim3(nblocks(ni, 4)), dim3(256), 0, st, b.part,
+                               nchunk, ni, b.cap, s.pid, b.out, b.nn, int(want_nn));
+        return hipGetLastError();
+    }
     if (prec == kF64)
         hipLaunchKernelGGL(k_reduce_f64, dim3(nblocks(ni, 256)), dim3(256), 0, st, b.part, nchunk,
                            ni, b.cap, s.pid, b.out, b.nn, int(want_nn));

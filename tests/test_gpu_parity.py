@@ -161,6 +161,35 @@ def test_tile_shapes_agree(g6):
     g6.set_tiles(0, 0)
 
 
+def test_many_chunk_reduction(g6):
+    """Small i-blocks against many j split over 16..79 chunks: the
+    one-wave-per-i reduction keeps FP64 parity and the exact nn."""
+    s = plummer_state(20000, 26)
+    upload(g6, s["x"], s["v"], s["a2"], s["j6"], s["k18"], s["m"], s["tj"], s["ids"])
+    g6.set_ti(s["ti"])
+    xp, vp = oracle.predict(s["ti"], s["tj"], s["x"], s["v"], s["a2"], s["j6"], s["k18"])
+    for ni in (1, 5, 64):
+        sel = np.arange(0, 20000, 20000 // ni)[:ni]
+        id_i, xi, vi = s["ids"][sel], xp[sel], vp[sel]
+        exact = oracle.force_ld(id_i, xi, vi, 1e-4, s["ids"], s["m"], xp, vp)
+        _, _, _, onn, _ = oracle.force(id_i, xi, vi, 1e-4, s["ids"], s["m"], xp, vp)
+        for prec in (0, 1):
+            g6.set_precision(prec)
+            for nchunk in (0, 16, 40, 79):
+                print_case(f"many chunks ni={ni} prec={prec} nchunk={nchunk}")
+                if nchunk:
+                    g6.set_tiles(256 if ni == 1 else 64, 1, nchunk, 0)
+                acc, jerk, pot, nn, _ = g6.calc(20000, id_i, xi, vi, 1e-4)
+                g6.set_tiles(0, 0)
+                if prec == 0:
+                    assert np.array_equal(nn, onn), (ni, nchunk)
+                    assert_plain_parity(acc, jerk, pot, exact)
+                else:
+                    d = np.abs(acc - exact[0]).max(1)
+                    assert (d <= 2e-6 * exact[3]).all(), (ni, nchunk)
+    g6.set_precision(0)
+
+
 def test_self_block_matches_abi_path(g6):
     s = plummer_state(2048, 23)
     upload(g6, s["x"], s["v"], s["a2"], s["j6"], s["k18"], s["m"], s["tj"], s["ids"])

@@ -85,3 +85,26 @@ def test_initial_energy_matches_direct_sum():
     h.close()
     assert k == pytest.approx(ic.kinetic_energy(m, v), rel=1e-14)
     assert u == pytest.approx(ic.potential_energy(m, x, 1e-4), rel=1e-12)
+
+
+def test_evolve_block_budget():
+    """evolve() reports whether t_end was reached or the block budget ran
+    out first; a run continued without a budget ends where an uninterrupted
+    run ends, bit for bit."""
+    oracle.build()
+    m, x, v = small_plummer(64, 6)
+    h = Hermite4(ORACLE_ABI)
+    h.commit(m, x, v, eps2=1e-4, eta=0.01)
+    assert h.evolve(0.0625, max_blocks=5) is False
+    assert h.counters()["block_steps"] == 5
+    assert h.evolve(0.0625) is True
+    xa, va = h.state()
+    ba = h.counters()["block_steps"]
+    h.close()
+    h = Hermite4(ORACLE_ABI)
+    h.commit(m, x, v, eps2=1e-4, eta=0.01)
+    assert h.evolve(0.0625) is True
+    xb, vb = h.state()
+    assert h.counters()["block_steps"] == ba
+    h.close()
+    assert np.array_equal(xa, xb) and np.array_equal(va, vb)
