@@ -648,17 +648,25 @@ G6C_API long long g6c_run(void *h, int steps_per_graph, int use_graph, long long
     hipEventCreate(&a);
     hipEventCreate(&b);
     hipEventRecord(a, E->stream);
+    long long rc = 0;
     for (;;) {
         if (max_rounds > 0 && rounds >= max_rounds) break;
+        hipError_t e = hipSuccess;
         if (use_graph) {
-            CK(hipGraphLaunch(E->graph, E->stream));
+            e = hipGraphLaunch(E->graph, E->stream);
         } else {
-            for (int s = 0; s < steps_per_graph; s++)
-                if (enqueue_step(E, E->precision)) return G6X_ERR_HIP;
+            for (int s = 0; s < steps_per_graph && e == hipSuccess; s++)
+                if (enqueue_step(E, E->precision)) e = hipErrorUnknown;
+        }
+        if (e == hipSuccess)
+            e = hipMemcpyAsync(h_active, E->d.nactive, sizeof(int), hipMemcpyDeviceToHost, E->stream);
+        if (e == hipSuccess) e = hipStreamSynchronize(E->stream);
+        if (e != hipSuccess) {
+            E->err = std::string("step rounds: ") + hipGetErrorString(e);
+            rc = G6X_ERR_HIP;
+            break;
         }
         rounds += steps_per_graph;
-        CK(hipMemcpyAsync(h_active, E->d.nactive, sizeof(int), hipMemcpyDeviceToHost, E->stream));
-        CK(hipStreamSynchronize(E->stream));
         if (*h_active <= 0) break;
     }
     hipEventRecord(b, E->stream);
@@ -669,7 +677,7 @@ G6C_API long long g6c_run(void *h, int steps_per_graph, int use_graph, long long
     hipEventDestroy(a);
     hipEventDestroy(b);
     hipHostFree(h_active);
-    return rounds;
+    return rc < 0 ? rc : rounds;
 }
 
 // Per-cluster total energy at the current state, from an FP64 force pass.

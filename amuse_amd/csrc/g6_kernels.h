@@ -29,7 +29,11 @@ constexpr int kTJ = 256;        // j per LDS tile
 constexpr int kPartFields = 9;   // mixed: (hi,lo) float pairs of acc3 jerk3 pot, r2, slot
 constexpr int kPartFieldsD = 16; // plain: hi, lo of acc3 jerk3 pot, r2, slot
 constexpr double kFar = 1.0e150;  // position given to empty slots
-constexpr int kReduceWaveMin = 16;  // chunks from which the reduce runs one wave per i
+// The reduce runs one wave per i from kReduceWaveMin chunks up to
+// kReduceWaveMaxNi i-particles (beyond that one lane per i has enough waves
+// in flight to hide the serial chunk loop).
+constexpr int kReduceWaveMin = 16;
+constexpr int kReduceWaveMaxNi = 8192;
 constexpr int kNgbCap = 256;      // neighbour ids kept per i (sapporo NGB_PP)
 
 struct JStore {

@@ -1177,7 +1177,7 @@ hipError_t launch_reduce(const JStore &s, const IBlock &b, int ni, int nchunk, i
                          bool want_nn, hipStream_t st)
 {
     if (ni <= 0 || nchunk <= 1) return hipSuccess;
-    if (nchunk >= kReduceWaveMin) {
+    if (nchunk >= kReduceWaveMin && ni <= kReduceWaveMaxNi) {
         // one wave per i, four i per workgroup
         if (prec == kF64)
             hipLaunchKernelGGL(k_reduce_f64_w, dim3(nblocks(ni, 4)), dim3(256), 0, st, b.part,

@@ -168,6 +168,7 @@ def blocks_study(args) -> int:
                 errs[q] = {"rel_max": float(rel.max()), "rel_p99": float(np.quantile(rel, .99)),
                            "cond_max": float(cond.max())}
             row[name] = {"kernel_ms": kms, "gpairs_per_s": ni * n / kms / 1e6,
+                         "reduce_us": tm["reduce_ms"] / tm["force_launches"] * 1e3,
                          "abi_call_us": call_us,
                          "predict_ms_total": pms, "errors": errs}
         rows.append(row)
@@ -306,9 +307,16 @@ def clusters_c2(args) -> int:
         tiles = ens.tiles()
         e0 = ens.energy()
         ens.start(1.0)
+        limit = args.max_rounds if args.max_rounds > 0 else 4000000
+        chunk = 4096  # step rounds per call, so progress is printed
         t0 = time.perf_counter()
-        rounds = ens.run(steps_per_graph=16, use_graph=True,
-                         max_rounds=args.max_rounds if args.max_rounds > 0 else 4000000)
+        rounds = 0
+        while rounds < limit:
+            r = ens.run(steps_per_graph=16, use_graph=True, max_rounds=min(chunk, limit - rounds))
+            rounds += r
+            log(f"  {name}: {rounds} step rounds, {time.perf_counter() - t0:.1f} s")
+            if r < chunk:
+                break
         wall = time.perf_counter() - t0
         e1 = ens.energy()
         _, _, tc, ns = ens.state()
@@ -363,7 +371,9 @@ def adjoint_c3(args) -> int:
 
     run(torch.float32)   # warm-up
     L32, g32, tf32, tb32 = run(torch.float32)
+    log(f"  c3 fp32: loss {L32:.6e} forward {tf32:.2f} s backward {tb32:.2f} s")
     L64, g64, tf64, tb64 = run(torch.float64)
+    log(f"  c3 fp64: loss {L64:.6e} forward {tf64:.2f} s backward {tb64:.2f} s")
     pairs_fwd = float(n) * n * (nsteps + 1)
     # the backward recomputes each checkpointed segment (forward pairs again)
     # and runs one adjoint pass per force evaluation
@@ -389,6 +399,7 @@ def adjoint_c3(args) -> int:
         an = float((g64[which] * d).sum())
         fd.append({"wrt": ["x0", "v0"][which], "analytic": an, "finite_diff": num,
                    "rel_err": abs(an - num) / max(abs(num), 1e-300)})
+        log(f"  c3 finite differences: {fd[-1]}")
     out = {"workload": f"c3: Plummer N={n}, {nsteps} Hermite steps dt=1/256, checkpoint {ck}, "
                        "Lagrangian-radius loss",
            "loss_fp32": L32, "loss_fp64": L64,
