@@ -403,6 +403,7 @@ struct Ensemble {
     hipGraphExec_t graph = nullptr;
     int graph_steps = 0;
     int graph_prec = -1;
+    int items_clusters = 0;  // clusters covered by the current work items
     double force_ms = 0.0;
     long long launches = 0;
     std::string err;
@@ -426,14 +427,19 @@ int size_class(int n)
 
 // Work items: every (cluster, i-block) pair, i-block = 256/njs rows, sorted
 // by descending cost (rows x cluster size) so the queue ends with the cheap
-// ones.
-int build_items(Ensemble *E)
+// ones.  `only` (host flags, one per cluster) restricts the list to some
+// clusters: every workgroup pulls items one atomic at a time, so items of
+// retired clusters would cost a serial chain of atomics per step.
+int build_items(Ensemble *E, const int *only = nullptr)
 {
     // the previous item arrays may still be read by queued work
     CK(hipStreamSynchronize(E->stream));
     std::vector<int> ic, ii0, injs;
     std::vector<double> cost;
+    int covered = 0;
     for (int c = 0; c < E->d.nc; c++) {
+        if (only && !only[c]) continue;
+        covered++;
         const int n = E->cnt[c];
         const int njs = E->njs_class[size_class(n)];
         const int rows = kWG / njs;
@@ -465,6 +471,7 @@ int build_items(Ensemble *E)
     CK(hipMemcpy(E->d.item_i0, b.data(), sizeof(int) * n, hipMemcpyHostToDevice));
     CK(hipMemcpy(E->d.item_njs, cc.data(), sizeof(int) * n, hipMemcpyHostToDevice));
     E->d.nitems = n;
+    E->items_clusters = covered;
     if (E->graph) {
         hipGraphExecDestroy(E->graph);
         E->graph = nullptr;
@@ -594,6 +601,8 @@ G6C_API const char *g6c_last_error(void *h) { return static_cast<Ensemble *>(h)-
 // per-cluster energies from their FP64 potential when precision is FP64.
 static int force_now(Ensemble *E, int precision)
 {
+    // a run may have dropped the items of retired clusters
+    if (E->items_clusters < E->d.nc && build_items(E)) return G6X_ERR_HIP;
     const int pb = (E->d.np + 255) / 256;
     hipLaunchKernelGGL(kc_predict, dim3(pb), dim3(256), 0, E->stream, E->d, 1);
     if (launch_force(E, precision)) return G6X_ERR_HIP;
@@ -618,6 +627,25 @@ G6C_API int g6c_start(void *h, double t_end)
     return 0;
 }
 
+static int capture_graph(Ensemble *E, int steps_per_graph)
+{
+    if (E->graph) hipGraphExecDestroy(E->graph);
+    E->graph = nullptr;
+    hipGraph_t g;
+    CK(hipStreamBeginCapture(E->stream, hipStreamCaptureModeThreadLocal));
+    for (int s = 0; s < steps_per_graph; s++)
+        if (enqueue_step(E, E->precision)) {
+            hipStreamEndCapture(E->stream, &g);
+            return G6X_ERR_HIP;
+        }
+    CK(hipStreamEndCapture(E->stream, &g));
+    CK(hipGraphInstantiate(&E->graph, g, nullptr, nullptr, 0));
+    hipGraphDestroy(g);
+    E->graph_steps = steps_per_graph;
+    E->graph_prec = E->precision;
+    return 0;
+}
+
 // Advances every cluster to t_end.  use_graph = 1 captures `steps_per_graph`
 // steps into one hipGraph and replays it; 0 launches the same kernels
 // eagerly.  Returns the number of step rounds executed.
@@ -625,24 +653,9 @@ G6C_API long long g6c_run(void *h, int steps_per_graph, int use_graph, long long
 {
     Ensemble *E = static_cast<Ensemble *>(h);
     if (steps_per_graph < 1) steps_per_graph = 1;
-    if (use_graph && (!E->graph || E->graph_steps != steps_per_graph || E->graph_prec != E->precision)) {
-        if (E->graph) hipGraphExecDestroy(E->graph);
-        E->graph = nullptr;
-        hipGraph_t g;
-        CK(hipStreamBeginCapture(E->stream, hipStreamCaptureModeThreadLocal));
-        for (int s = 0; s < steps_per_graph; s++)
-            if (enqueue_step(E, E->precision)) {
-                hipStreamEndCapture(E->stream, &g);
-                return G6X_ERR_HIP;
-            }
-        CK(hipStreamEndCapture(E->stream, &g));
-        CK(hipGraphInstantiate(&E->graph, g, nullptr, nullptr, 0));
-        hipGraphDestroy(g);
-        E->graph_steps = steps_per_graph;
-        E->graph_prec = E->precision;
-    }
     int *h_active = nullptr;
     CK(hipHostMalloc(reinterpret_cast<void **>(&h_active), sizeof(int)));
+    std::vector<int> flags(E->d.nc);
     long long rounds = 0;
     hipEvent_t a, b;
     hipEventCreate(&a);
@@ -653,6 +666,8 @@ G6C_API long long g6c_run(void *h, int steps_per_graph, int use_graph, long long
         if (max_rounds > 0 && rounds >= max_rounds) break;
         hipError_t e = hipSuccess;
         if (use_graph) {
+            if (!E->graph || E->graph_steps != steps_per_graph || E->graph_prec != E->precision)
+                if (capture_graph(E, steps_per_graph)) { rc = G6X_ERR_HIP; break; }
             e = hipGraphLaunch(E->graph, E->stream);
         } else {
             for (int s = 0; s < steps_per_graph && e == hipSuccess; s++)
@@ -668,6 +683,15 @@ G6C_API long long g6c_run(void *h, int steps_per_graph, int use_graph, long long
         }
         rounds += steps_per_graph;
         if (*h_active <= 0) break;
+        if (*h_active * 2 <= E->items_clusters) {
+            // half the covered clusters have retired: drop their items
+            e = hipMemcpy(flags.data(), E->d.active, sizeof(int) * E->d.nc, hipMemcpyDeviceToHost);
+            if (e != hipSuccess || build_items(E, flags.data())) {
+                if (e != hipSuccess) E->err = std::string("active flags: ") + hipGetErrorString(e);
+                rc = G6X_ERR_HIP;
+                break;
+            }
+        }
     }
     hipEventRecord(b, E->stream);
     hipEventSynchronize(b);
@@ -782,7 +806,7 @@ G6C_API int g6c_autotune(void *h)
         int best_njs = E->njs_class[k];
         for (int njs = 1; njs <= 64; njs *= 2) {
             E->njs_class[k] = njs;
-            if (build_items(E)) return G6X_ERR_HIP;
+            if (build_items(E, act.data())) return G6X_ERR_HIP;
             hipLaunchKernelGGL(kc_predict, dim3((E->d.np + 255) / 256), dim3(256), 0, E->stream, E->d, 1);
             launch_force(E, E->precision);
             hipEventRecord(a, E->stream);

@@ -76,6 +76,72 @@ struct ForceCfg {
     bool smem = false;  // njs == 1 only: j read through scalar loads, no LDS
 };
 
+// ------------------------------------------------------------------------
+// Block steps decided on the device (the device-resident driver,
+// g6_blocks.hip).  The active count of a step exists only on the GPU, so
+// the predictor, force and reduce kernels of a step read it from this
+// control block and size their work themselves; one captured launch
+// sequence then serves every step.
+struct StepCtl {
+    unsigned long long tmin;  // bits of min(t + dt) (positive doubles order as integers)
+    int ni;                   // active particles of the current step
+    int done;                 // 1: t_end reached or block budget spent
+    long long blocks, psteps; // block and particle steps taken
+    long long budget;         // block steps left in the current run (< 0: no limit)
+    double t_end;
+    double system_time;
+};
+
+struct StepPlan {
+    int shape;    // 0..3: (lanes per i, i per lane) = (1,2) (4,1) (16,1) (64,1)
+    int iblocks;  // i-blocks of the shape's width
+    int nchunk;   // j-chunks (partials reduced when > 1)
+    int jchunk;   // j per chunk (multiple of kTJ)
+};
+
+constexpr long long kStepPartBudget = 1ll << 21;  // nchunk * ni bound on partials
+constexpr int kStepGrid = 2048;                   // persistent workgroups of a step
+
+__host__ __device__ inline int step_ib(int shape)
+{
+    return shape == 0 ? 512 : shape == 1 ? 64 : shape == 2 ? 16 : 4;
+}
+
+// Same rules as the host-side choice for the ABI path, over four shapes:
+// the widest i-block not wider than the next power of two of ni, then
+// j-chunks for ~8 workgroups per CU within the partials budget.
+__host__ __device__ inline StepPlan plan_step(int ni, int nj)
+{
+    StepPlan sp;
+    int p2 = 1;
+    while (p2 < ni) p2 *= 2;
+    sp.shape = 3;
+    for (int k = 0; k < 4; k++)
+        if (step_ib(k) <= p2) { sp.shape = k; break; }
+    const int ib = step_ib(sp.shape);
+    sp.iblocks = (ni + ib - 1) / ib;
+    const int max_chunks = nj > kTJ ? (nj + kTJ - 1) / kTJ : 1;
+    int nchunk = sp.iblocks > 0 ? (2048 + sp.iblocks - 1) / sp.iblocks : 1;
+    if (nchunk > max_chunks) nchunk = max_chunks;
+    while (nchunk > 1 && (long long)nchunk * ni > kStepPartBudget) nchunk /= 2;
+    int jchunk = (nj + nchunk - 1) / nchunk;
+    jchunk = ((jchunk + kTJ - 1) / kTJ) * kTJ;
+    if (jchunk <= 0) jchunk = kTJ;
+    sp.jchunk = jchunk;
+    sp.nchunk = nj > 0 ? (nj + jchunk - 1) / jchunk : 1;
+    return sp;
+}
+
+// Predicts every j-slot to the step's block time (at_system = 0) or to
+// c->system_time (1); no-op once the step is done.
+hipError_t launch_step_predict(const JStore &s, int nj, const StepCtl *c, int at_system,
+                               int prec, int compat, hipStream_t st);
+// Forces (acc, jerk, pot; no nn) of the step's i-block (b rows 0..ni-1,
+// ni = c->ni) against j-slots [0, nj) into b.out, with partials in part
+// (capacity kStepPartBudget * fields doubles) reduced by a second kernel.
+hipError_t launch_step_force(const JStore &s, const IBlock &b, double *part, int nj,
+                             double eps2, int prec, const StepCtl *c, hipStream_t st);
+
 size_t jstore_bytes(int cap);
 void jstore_carve(JStore &s, void *base, int cap);
 size_t iblock_bytes(int cap);

@@ -375,7 +375,8 @@ struct FParams {
     double *out;  // [7][ostride]
     int *nn;
     int ostride;
-    double *part;  // [chunk][9][ostride]
+    double *part;  // [chunk][fields][pstride]
+    int pstride;
 };
 
 // Shared-memory layout of one j tile.
@@ -388,19 +389,21 @@ struct TileF {
     float4 c[kTJ];  // vy vz id -
 };
 
+// One (i-block bx, j-chunk by) work item; the kernel below runs one item
+// per workgroup, kd_* below loops over items whose count lives on the device.
 template <int NJS, int IPL, bool NN>
-__global__ void __launch_bounds__(kWG) k_force_f64(FParams p)
+__device__ __forceinline__ void force_f64_item(const FParams &p, TileD &tile, int bx, int by,
+                                       int nchunk)
 {
     constexpr int IBL = kWG / NJS;
     constexpr int IB = IBL * IPL;
     // tiles between two folds of the plain sums: >= 128 pairs per lane
     constexpr int FT = NJS >= 4 ? NJS / 2 : 1;
-    __shared__ TileD tile;
 
     const int t = threadIdx.x;
     const int js = t % NJS;
     const int il = t / NJS;
-    const int ibase = blockIdx.x * IB;
+    const int ibase = bx * IB;
 
     ID I[IPL];
     AccD A[IPL];
@@ -416,7 +419,7 @@ __global__ void __launch_bounds__(kWG) k_force_f64(FParams p)
         for (int k = 0; k < 7; k++) { D[q].h[k] = 0.0; D[q].l[k] = 0.0; }
     }
     const double eps2 = p.eps2;
-    const int j0 = blockIdx.y * p.jchunk;
+    const int j0 = by * p.jchunk;
     const int j1 = min(p.nj, j0 + p.jchunk);
 
     int tcount = 0;
@@ -505,22 +508,30 @@ __global__ void __launch_bounds__(kWG) k_force_f64(FParams p)
         const int i = ibase + il + q * IBL;
         if (i >= p.ni) continue;
         const int S = p.ostride;
-        if (gridDim.y == 1) {
+        if (nchunk == 1) {
 #pragma unroll
             for (int k = 0; k < 7; k++) p.out[k * S + i] = D[q].h[k] + D[q].l[k];
             if (NN) p.nn[i] = A[q].addr >= 0 ? p.jid[A[q].addr] : -1;
         } else {
             // partial record: hi and lo of each sum in consecutive field pairs
-            double *r = p.part + size_t(blockIdx.y) * kPartFieldsD * S;
+            const int PS = p.pstride;
+            double *r = p.part + size_t(by) * kPartFieldsD * PS;
 #pragma unroll
             for (int k = 0; k < 7; k++) {
-                r[(2 * k) * S + i] = D[q].h[k];
-                r[(2 * k + 1) * S + i] = D[q].l[k];
+                r[(2 * k) * PS + i] = D[q].h[k];
+                r[(2 * k + 1) * PS + i] = D[q].l[k];
             }
-            r[14 * S + i] = NN ? A[q].best : __builtin_inf();
-            r[15 * S + i] = NN ? double(A[q].addr) : -1.0;
+            r[14 * PS + i] = NN ? A[q].best : __builtin_inf();
+            r[15 * PS + i] = NN ? double(A[q].addr) : -1.0;
         }
     }
+}
+
+template <int NJS, int IPL, bool NN>
+__global__ void __launch_bounds__(kWG) k_force_f64(FParams p)
+{
+    __shared__ TileD tile;
+    force_f64_item<NJS, IPL, NN>(p, tile, blockIdx.x, blockIdx.y, gridDim.y);
 }
 
 // ------------------------------------------------------------------------
@@ -602,17 +613,19 @@ __device__ __forceinline__ void flush_mx(AccF &a, DSAcc &d)
 #endif
 constexpr int kFlush = G6_MX_FLUSH;
 
+// One (i-block bx, j-chunk by) work item; the kernel below runs one item
+// per workgroup, kd_* below loops over items whose count lives on the device.
 template <int NJS, int IPL, bool NN>
-__global__ void __launch_bounds__(kWG) k_force_mx(FParams p)
+__device__ __forceinline__ void force_mx_item(const FParams &p, TileF &tile, int bx, int by,
+                                       int nchunk)
 {
     constexpr int IBL = kWG / NJS;
     constexpr int IB = IBL * IPL;
-    __shared__ TileF tile;
 
     const int t = threadIdx.x;
     const int js = t % NJS;
     const int il = t / NJS;
-    const int ibase = blockIdx.x * IB;
+    const int ibase = bx * IB;
 
     IF I[IPL];
     AccF A[IPL];
@@ -631,7 +644,7 @@ __global__ void __launch_bounds__(kWG) k_force_mx(FParams p)
         for (int k = 0; k < 7; k++) { D[q].h[k] = 0.0f; D[q].l[k] = 0.0f; }
     }
     const float eps2 = float(p.eps2);
-    const int j0 = blockIdx.y * p.jchunk;
+    const int j0 = by * p.jchunk;
     const int j1 = min(p.nj, j0 + p.jchunk);
 
     for (int tb = j0; tb < j1; tb += kTJ) {
@@ -713,19 +726,27 @@ __global__ void __launch_bounds__(kWG) k_force_mx(FParams p)
         const int i = ibase + il + q * IBL;
         if (i >= p.ni) continue;
         const int S = p.ostride;
-        if (gridDim.y == 1) {
+        if (nchunk == 1) {
 #pragma unroll
             for (int k = 0; k < 7; k++) p.out[k * S + i] = double(D[q].h[k]) + double(D[q].l[k]);
             if (NN) p.nn[i] = A[q].addr >= 0 ? p.jid[A[q].addr] : -1;
         } else {
             // partial record: (hi, lo) float pairs packed into the double slots
-            float2 *r = reinterpret_cast<float2 *>(p.part + size_t(blockIdx.y) * kPartFields * S);
+            const int PS = p.pstride;
+            float2 *r = reinterpret_cast<float2 *>(p.part + size_t(by) * kPartFields * PS);
 #pragma unroll
-            for (int k = 0; k < 7; k++) r[k * S + i] = make_float2(D[q].h[k], D[q].l[k]);
-            r[7 * S + i] = make_float2(NN ? A[q].best : __builtin_inff(), 0.0f);
-            r[8 * S + i] = make_float2(__int_as_float(NN ? A[q].addr : -1), 0.0f);
+            for (int k = 0; k < 7; k++) r[k * PS + i] = make_float2(D[q].h[k], D[q].l[k]);
+            r[7 * PS + i] = make_float2(NN ? A[q].best : __builtin_inff(), 0.0f);
+            r[8 * PS + i] = make_float2(__int_as_float(NN ? A[q].addr : -1), 0.0f);
         }
     }
+}
+
+template <int NJS, int IPL, bool NN>
+__global__ void __launch_bounds__(kWG) k_force_mx(FParams p)
+{
+    __shared__ TileF tile;
+    force_mx_item<NJS, IPL, NN>(p, tile, blockIdx.x, blockIdx.y, gridDim.y);
 }
 
 // ------------------------------------------------------------------------
@@ -781,14 +802,15 @@ __global__ void __launch_bounds__(kWG) k_force_f64_s(FParams p)
             for (int k = 0; k < 7; k++) p.out[k * S + i] = D[q].h[k] + D[q].l[k];
             if (NN) p.nn[i] = A[q].addr >= 0 ? p.jid[A[q].addr] : -1;
         } else {
-            double *r = p.part + size_t(blockIdx.y) * kPartFieldsD * S;
+            const int PS = p.pstride;
+            double *r = p.part + size_t(blockIdx.y) * kPartFieldsD * PS;
 #pragma unroll
             for (int k = 0; k < 7; k++) {
-                r[(2 * k) * S + i] = D[q].h[k];
-                r[(2 * k + 1) * S + i] = D[q].l[k];
+                r[(2 * k) * PS + i] = D[q].h[k];
+                r[(2 * k + 1) * PS + i] = D[q].l[k];
             }
-            r[14 * S + i] = NN ? A[q].best : __builtin_inf();
-            r[15 * S + i] = NN ? double(A[q].addr) : -1.0;
+            r[14 * PS + i] = NN ? A[q].best : __builtin_inf();
+            r[15 * PS + i] = NN ? double(A[q].addr) : -1.0;
         }
     }
 }
@@ -841,97 +863,79 @@ __global__ void __launch_bounds__(kWG) k_force_mx_s(FParams p)
             for (int k = 0; k < 7; k++) p.out[k * S + i] = double(D[q].h[k]) + double(D[q].l[k]);
             if (NN) p.nn[i] = A[q].addr >= 0 ? p.jid[A[q].addr] : -1;
         } else {
-            float2 *r = reinterpret_cast<float2 *>(p.part + size_t(blockIdx.y) * kPartFields * S);
+            const int PS = p.pstride;
+            float2 *r = reinterpret_cast<float2 *>(p.part + size_t(blockIdx.y) * kPartFields * PS);
 #pragma unroll
-            for (int k = 0; k < 7; k++) r[k * S + i] = make_float2(D[q].h[k], D[q].l[k]);
-            r[7 * S + i] = make_float2(NN ? A[q].best : __builtin_inff(), 0.0f);
-            r[8 * S + i] = make_float2(__int_as_float(NN ? A[q].addr : -1), 0.0f);
+            for (int k = 0; k < 7; k++) r[k * PS + i] = make_float2(D[q].h[k], D[q].l[k]);
+            r[7 * PS + i] = make_float2(NN ? A[q].best : __builtin_inff(), 0.0f);
+            r[8 * PS + i] = make_float2(__int_as_float(NN ? A[q].addr : -1), 0.0f);
         }
     }
 }
 
 // ------------------------------------------------------------------------
-// Ordered reduction of the per-chunk partials (one lane per i).
+// Ordered reduction of the per-chunk partials.  Partials are
+// [chunk][field][PS], results [7][S] (+ nn).  Two forms: one lane folds all
+// chunks of its i in chunk order; or one wave per i, lane L folding chunks
+// L, L + 64, ... followed by a fixed butterfly over the lanes (for small
+// i-blocks with many chunks, where the serial loop is a latency chain).
+// Either way the result does not depend on timing.
 // ------------------------------------------------------------------------
-__global__ void __launch_bounds__(256) k_reduce_f64(const double *__restrict__ part, int nchunk,
-                                                    int ni, int S, const int *__restrict__ jid,
-                                                    double *__restrict__ out, int *__restrict__ nn,
-                                                    int want_nn)
+__device__ __forceinline__ void fold_chunk_f64(const double *__restrict__ part, int ch, int PS,
+                                               int i, double *h, double *l, double &best,
+                                               int &addr, int want_nn)
 {
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= ni) return;
-    double h[7], l[7];
+    const double *r = part + size_t(ch) * kPartFieldsD * PS;
+#pragma unroll
+    for (int k = 0; k < 7; k++) {
+        two_sum_add_d(h[k], l[k], r[(2 * k) * PS + i]);
+        l[k] += r[(2 * k + 1) * PS + i];
+    }
+    if (want_nn) nn_merge(best, addr, r[14 * PS + i], int(r[15 * PS + i]));
+}
+
+__device__ __forceinline__ void fold_chunk_mx(const double *__restrict__ part, int ch, int PS,
+                                              int i, float *h, float *l, float &best, int &addr,
+                                              int want_nn)
+{
+    const float2 *r = reinterpret_cast<const float2 *>(part + size_t(ch) * kPartFields * PS);
+#pragma unroll
+    for (int k = 0; k < 7; k++) {
+        const float2 v = r[k * PS + i];
+        two_sum_add(h[k], l[k], v.x);
+        l[k] += v.y;
+    }
+    if (want_nn) nn_merge_f(best, addr, r[7 * PS + i].x, __float_as_int(r[8 * PS + i].x));
+}
+
+// lane form (first == 0, step == 1) or one lane of the wave form
+// (first = lane, step = 64, followed by butterfly_*)
+__device__ __forceinline__ void reduce_i_f64(const double *__restrict__ part, int nchunk, int PS,
+                                             int i, int first, int step, double *h, double *l,
+                                             double &best, int &addr, int want_nn)
+{
 #pragma unroll
     for (int k = 0; k < 7; k++) { h[k] = 0.0; l[k] = 0.0; }
-    double best = __builtin_inf();
-    int addr = -1;
-    for (int ch = 0; ch < nchunk; ch++) {
-        const double *r = part + size_t(ch) * kPartFieldsD * S;
-#pragma unroll
-        for (int k = 0; k < 7; k++) {
-            two_sum_add_d(h[k], l[k], r[(2 * k) * S + i]);
-            l[k] += r[(2 * k + 1) * S + i];
-        }
-        if (want_nn) nn_merge(best, addr, r[14 * S + i], int(r[15 * S + i]));
-    }
-#pragma unroll
-    for (int k = 0; k < 7; k++) out[k * S + i] = h[k] + l[k];
-    if (want_nn) nn[i] = addr >= 0 ? jid[addr] : -1;
+    best = __builtin_inf();
+    addr = -1;
+    for (int ch = first; ch < nchunk; ch += step)
+        fold_chunk_f64(part, ch, PS, i, h, l, best, addr, want_nn);
 }
 
-__global__ void __launch_bounds__(256) k_reduce_mx(const double *__restrict__ part, int nchunk,
-                                                   int ni, int S, const int *__restrict__ jid,
-                                                   double *__restrict__ out, int *__restrict__ nn,
-                                                   int want_nn)
+__device__ __forceinline__ void reduce_i_mx(const double *__restrict__ part, int nchunk, int PS,
+                                            int i, int first, int step, float *h, float *l,
+                                            float &best, int &addr, int want_nn)
 {
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= ni) return;
-    float h[7], l[7];
 #pragma unroll
     for (int k = 0; k < 7; k++) { h[k] = 0.0f; l[k] = 0.0f; }
-    float best = __builtin_inff();
-    int addr = -1;
-    for (int ch = 0; ch < nchunk; ch++) {
-        const float2 *r = reinterpret_cast<const float2 *>(part + size_t(ch) * kPartFields * S);
-#pragma unroll
-        for (int k = 0; k < 7; k++) {
-            const float2 v = r[k * S + i];
-            two_sum_add(h[k], l[k], v.x);
-            l[k] += v.y;
-        }
-        if (want_nn) nn_merge_f(best, addr, r[7 * S + i].x, __float_as_int(r[8 * S + i].x));
-    }
-#pragma unroll
-    for (int k = 0; k < 7; k++) out[k * S + i] = double(h[k]) + double(l[k]);
-    if (want_nn) nn[i] = addr >= 0 ? jid[addr] : -1;
+    best = __builtin_inff();
+    addr = -1;
+    for (int ch = first; ch < nchunk; ch += step)
+        fold_chunk_mx(part, ch, PS, i, h, l, best, addr, want_nn);
 }
 
-// One wave per i for many chunks: lane L folds chunks L, L + 64, ... and a
-// fixed butterfly combines the lanes, so the result does not depend on
-// timing (the summation order differs from the one-lane form, which keeps
-// the latency chain of a long serial loop for small i-blocks).
-__global__ void __launch_bounds__(256) k_reduce_f64_w(const double *__restrict__ part, int nchunk,
-                                                      int ni, int S, const int *__restrict__ jid,
-                                                      double *__restrict__ out,
-                                                      int *__restrict__ nn, int want_nn)
+__device__ __forceinline__ void butterfly_f64(double *h, double *l, double &best, int &addr)
 {
-    const int i = blockIdx.x * 4 + int(threadIdx.x >> 6);
-    const int lane = threadIdx.x & 63;
-    if (i >= ni) return;  // whole waves leave together
-    double h[7], l[7];
-#pragma unroll
-    for (int k = 0; k < 7; k++) { h[k] = 0.0; l[k] = 0.0; }
-    double best = __builtin_inf();
-    int addr = -1;
-    for (int ch = lane; ch < nchunk; ch += 64) {
-        const double *r = part + size_t(ch) * kPartFieldsD * S;
-#pragma unroll
-        for (int k = 0; k < 7; k++) {
-            two_sum_add_d(h[k], l[k], r[(2 * k) * S + i]);
-            l[k] += r[(2 * k + 1) * S + i];
-        }
-        if (want_nn) nn_merge(best, addr, r[14 * S + i], int(r[15 * S + i]));
-    }
 #pragma unroll
     for (int off = 32; off >= 1; off >>= 1) {
 #pragma unroll
@@ -944,35 +948,10 @@ __global__ void __launch_bounds__(256) k_reduce_f64_w(const double *__restrict__
         const int oa = __shfl_xor(addr, off);
         nn_merge(best, addr, ob, oa);
     }
-    if (lane != 0) return;
-#pragma unroll
-    for (int k = 0; k < 7; k++) out[k * S + i] = h[k] + l[k];
-    if (want_nn) nn[i] = addr >= 0 ? jid[addr] : -1;
 }
 
-__global__ void __launch_bounds__(256) k_reduce_mx_w(const double *__restrict__ part, int nchunk,
-                                                     int ni, int S, const int *__restrict__ jid,
-                                                     double *__restrict__ out,
-                                                     int *__restrict__ nn, int want_nn)
+__device__ __forceinline__ void butterfly_mx(float *h, float *l, float &best, int &addr)
 {
-    const int i = blockIdx.x * 4 + int(threadIdx.x >> 6);
-    const int lane = threadIdx.x & 63;
-    if (i >= ni) return;
-    float h[7], l[7];
-#pragma unroll
-    for (int k = 0; k < 7; k++) { h[k] = 0.0f; l[k] = 0.0f; }
-    float best = __builtin_inff();
-    int addr = -1;
-    for (int ch = lane; ch < nchunk; ch += 64) {
-        const float2 *r = reinterpret_cast<const float2 *>(part + size_t(ch) * kPartFields * S);
-#pragma unroll
-        for (int k = 0; k < 7; k++) {
-            const float2 v = r[k * S + i];
-            two_sum_add(h[k], l[k], v.x);
-            l[k] += v.y;
-        }
-        if (want_nn) nn_merge_f(best, addr, r[7 * S + i].x, __float_as_int(r[8 * S + i].x));
-    }
 #pragma unroll
     for (int off = 32; off >= 1; off >>= 1) {
 #pragma unroll
@@ -985,10 +964,139 @@ __global__ void __launch_bounds__(256) k_reduce_mx_w(const double *__restrict__ 
         const int oa = __shfl_xor(addr, off);
         nn_merge_f(best, addr, ob, oa);
     }
-    if (lane != 0) return;
+}
+
+template <class T>
+__device__ __forceinline__ void store_reduced(const T *h, const T *l, int addr, int S, int i,
+                                              const int *__restrict__ jid,
+                                              double *__restrict__ out, int *__restrict__ nn,
+                                              int want_nn)
+{
 #pragma unroll
     for (int k = 0; k < 7; k++) out[k * S + i] = double(h[k]) + double(l[k]);
     if (want_nn) nn[i] = addr >= 0 ? jid[addr] : -1;
+}
+
+template <bool MX>
+__device__ __forceinline__ void reduce_one(const double *__restrict__ part, int nchunk, int PS,
+                                           int S, int i, bool wave, const int *__restrict__ jid,
+                                           double *__restrict__ out, int *__restrict__ nn,
+                                           int want_nn)
+{
+    const int lane = threadIdx.x & 63;
+    const int first = wave ? lane : 0, step = wave ? 64 : 1;
+    int addr;
+    if constexpr (MX) {
+        float h[7], l[7], best;
+        reduce_i_mx(part, nchunk, PS, i, first, step, h, l, best, addr, want_nn);
+        if (wave) butterfly_mx(h, l, best, addr);
+        if (!wave || lane == 0) store_reduced(h, l, addr, S, i, jid, out, nn, want_nn);
+    } else {
+        double h[7], l[7], best;
+        reduce_i_f64(part, nchunk, PS, i, first, step, h, l, best, addr, want_nn);
+        if (wave) butterfly_f64(h, l, best, addr);
+        if (!wave || lane == 0) store_reduced(h, l, addr, S, i, jid, out, nn, want_nn);
+    }
+}
+
+template <bool MX, bool WAVE>
+__global__ void __launch_bounds__(256) k_reduce(const double *__restrict__ part, int nchunk,
+                                                int ni, int PS, int S,
+                                                const int *__restrict__ jid,
+                                                double *__restrict__ out, int *__restrict__ nn,
+                                                int want_nn)
+{
+    // lane form: one i per thread; wave form: one i per wave, four per workgroup
+    const int i = WAVE ? int(blockIdx.x * 4 + (threadIdx.x >> 6))
+                       : int(blockIdx.x * blockDim.x + threadIdx.x);
+    if (i >= ni) return;  // wave form: whole waves leave together
+    reduce_one<MX>(part, nchunk, PS, S, i, WAVE, jid, out, nn, want_nn);
+}
+
+// ------------------------------------------------------------------------
+// Device-decided block steps: the kernels read the active count of the step
+// from the control block and plan their own work (plan_step), so a captured
+// sequence of them serves steps of any size.  Grids are fixed; workgroups
+// loop over work items and leave when there are none (ni = 0 or done).
+// ------------------------------------------------------------------------
+__device__ __forceinline__ double step_time(const StepCtl *c, int at_system)
+{
+    return at_system ? c->system_time
+                     : __longlong_as_double(static_cast<long long>(c->tmin));
+}
+
+__global__ void __launch_bounds__(256) kd_predict(JStore s, int nj, const StepCtl *c,
+                                                  int at_system, int prec, int compat)
+{
+    if (c->done) return;
+    const double ti = step_time(c, at_system);
+    for (int j = blockIdx.x * blockDim.x + threadIdx.x; j < nj; j += gridDim.x * blockDim.x) {
+        double x[3], v[3], a2[3], j6[3];
+        const double k18[3] = {0.0, 0.0, 0.0};
+#pragma unroll
+        for (int k = 0; k < 3; k++) {
+            x[k] = s.x[k][j]; v[k] = s.v[k][j]; a2[k] = s.a2[k][j]; j6[k] = s.j6[k][j];
+        }
+        const Pred pr = predict_slot(ti, s.tj[j], x, v, a2, j6, k18, compat, false);
+        write_pred(s, j, prec, s.id[j], pr);
+    }
+}
+
+template <bool MX>
+__global__ void __launch_bounds__(kWG) kd_force(FParams p, const StepCtl *c)
+{
+    if (c->done) return;
+    const int ni = c->ni;
+    if (ni <= 0) return;
+    const StepPlan sp = plan_step(ni, p.nj);
+    p.ni = ni;
+    p.jchunk = sp.jchunk;
+    p.pstride = ni;
+    const int items = sp.iblocks * sp.nchunk;
+    if constexpr (MX) {
+        __shared__ TileF tile;
+        for (int w = blockIdx.x; w < items; w += gridDim.x) {
+            const int bx = w % sp.iblocks, by = w / sp.iblocks;
+            switch (sp.shape) {
+            case 0: force_mx_item<1, 2, false>(p, tile, bx, by, sp.nchunk); break;
+            case 1: force_mx_item<4, 1, false>(p, tile, bx, by, sp.nchunk); break;
+            case 2: force_mx_item<16, 1, false>(p, tile, bx, by, sp.nchunk); break;
+            default: force_mx_item<64, 1, false>(p, tile, bx, by, sp.nchunk); break;
+            }
+        }
+    } else {
+        __shared__ TileD tile;
+        for (int w = blockIdx.x; w < items; w += gridDim.x) {
+            const int bx = w % sp.iblocks, by = w / sp.iblocks;
+            switch (sp.shape) {
+            case 0: force_f64_item<1, 2, false>(p, tile, bx, by, sp.nchunk); break;
+            case 1: force_f64_item<4, 1, false>(p, tile, bx, by, sp.nchunk); break;
+            case 2: force_f64_item<16, 1, false>(p, tile, bx, by, sp.nchunk); break;
+            default: force_f64_item<64, 1, false>(p, tile, bx, by, sp.nchunk); break;
+            }
+        }
+    }
+}
+
+template <bool MX>
+__global__ void __launch_bounds__(256) kd_reduce(const double *__restrict__ part,
+                                                 const StepCtl *c, int nj, int S,
+                                                 const int *__restrict__ jid,
+                                                 double *__restrict__ out)
+{
+    if (c->done) return;
+    const int ni = c->ni;
+    if (ni <= 0) return;
+    const StepPlan sp = plan_step(ni, nj);
+    if (sp.nchunk <= 1) return;
+    if (sp.nchunk >= kReduceWaveMin && ni <= kReduceWaveMaxNi) {
+        // one wave per i; the loop bound is wave-uniform
+        for (int i = blockIdx.x * 4 + int(threadIdx.x >> 6); i < ni; i += gridDim.x * 4)
+            reduce_one<MX>(part, sp.nchunk, ni, S, i, true, jid, out, nullptr, 0);
+    } else {
+        for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < ni; i += gridDim.x * blockDim.x)
+            reduce_one<MX>(part, sp.nchunk, ni, S, i, false, jid, out, nullptr, 0);
+    }
 }
 
 // ------------------------------------------------------------------------
@@ -1141,6 +1249,7 @@ hipError_t launch_force(const JStore &s, const IBlock &b, int ni, int nj, double
     fp.out = b.out;
     fp.nn = b.nn;
     fp.ostride = b.cap;
+    fp.pstride = b.cap;
     fp.part = b.part;
     const int ib = (kWG / cfg.njs) * cfg.ipl;
     dim3 grid(nblocks(ni, ib), cfg.nchunk);
@@ -1177,22 +1286,72 @@ hipError_t launch_reduce(const JStore &s, const IBlock &b, int ni, int nchunk, i
                          bool want_nn, hipStream_t st)
 {
     if (ni <= 0 || nchunk <= 1) return hipSuccess;
+    const int S = b.cap;
     if (nchunk >= kReduceWaveMin && ni <= kReduceWaveMaxNi) {
-        // one wave per i, four i per workgroup
         if (prec == kF64)
-            hipLaunchKernelGGL(k_reduce_f64_w, dim3(nblocks(ni, 4)), dim3(256), 0, st, b.part,
-                               nchunk, ni, b.cap, s.pid, b.out, b.nn, int(want_nn));
+            hipLaunchKernelGGL((k_reduce<false, true>), dim3(nblocks(ni, 4)), dim3(256), 0, st,
+                               b.part, nchunk, ni, S, S, s.pid, b.out, b.nn, int(want_nn));
         else
-            hipLaunchKernelGGL(k_reduce_mx_w, dim3(nblocks(ni, 4)), dim3(256), 0, st, b.part,
-                               nchunk, ni, b.cap, s.pid, b.out, b.nn, int(want_nn));
+            hipLaunchKernelGGL((k_reduce<true, true>), dim3(nblocks(ni, 4)), dim3(256), 0, st,
+                               b.part, nchunk, ni, S, S, s.pid, b.out, b.nn, int(want_nn));
         return hipGetLastError();
     }
     if (prec == kF64)
-        hipLaunchKernelGGL(k_reduce_f64, dim3(nblocks(ni, 256)), dim3(256), 0, st, b.part, nchunk,
-                           ni, b.cap, s.pid, b.out, b.nn, int(want_nn));
+        hipLaunchKernelGGL((k_reduce<false, false>), dim3(nblocks(ni, 256)), dim3(256), 0, st,
+                           b.part, nchunk, ni, S, S, s.pid, b.out, b.nn, int(want_nn));
     else
-        hipLaunchKernelGGL(k_reduce_mx, dim3(nblocks(ni, 256)), dim3(256), 0, st, b.part, nchunk,
-                           ni, b.cap, s.pid, b.out, b.nn, int(want_nn));
+        hipLaunchKernelGGL((k_reduce<true, false>), dim3(nblocks(ni, 256)), dim3(256), 0, st,
+                           b.part, nchunk, ni, S, S, s.pid, b.out, b.nn, int(want_nn));
+    return hipGetLastError();
+}
+
+hipError_t launch_step_predict(const JStore &s, int nj, const StepCtl *c, int at_system,
+                               int prec, int compat, hipStream_t st)
+{
+    if (nj <= 0) return hipSuccess;
+    const int blocks = std::min(nblocks(nj, 256), 4096);
+    hipLaunchKernelGGL(kd_predict, dim3(blocks), dim3(256), 0, st, s, nj, c, at_system, prec,
+                       compat);
+    return hipGetLastError();
+}
+
+hipError_t launch_step_force(const JStore &s, const IBlock &b, double *part, int nj,
+                             double eps2, int prec, const StepCtl *c, hipStream_t st)
+{
+    if (nj <= 0) return hipSuccess;
+    FParams fp{};
+    fp.ix = b.x[0]; fp.iy = b.x[1]; fp.iz = b.x[2];
+    fp.ivx = b.v[0]; fp.ivy = b.v[1]; fp.ivz = b.v[2];
+    fp.iid = b.id;
+    fp.jx = s.px[0]; fp.jy = s.px[1]; fp.jz = s.px[2]; fp.jm = s.pm;
+    fp.jvx = s.pv[0]; fp.jvy = s.pv[1]; fp.jvz = s.pv[2];
+    fp.hx = s.hx[0]; fp.hy = s.hx[1]; fp.hz = s.hx[2];
+    fp.lx = s.lx[0]; fp.ly = s.lx[1]; fp.lz = s.lx[2];
+    fp.fm = s.fm; fp.fvx = s.fv[0]; fp.fvy = s.fv[1]; fp.fvz = s.fv[2];
+    fp.jid = s.pid;
+    fp.nj = nj;
+    fp.eps2 = eps2;
+    fp.out = b.out;
+    fp.nn = b.nn;
+    fp.ostride = b.cap;
+    fp.part = part;
+    // Grids sized for the largest step of this system (ni <= nj): a plan has
+    // iblocks * nchunk <= 2048 + iblocks items, nchunk <= nj / kTJ, and more
+    // than 4 i-blocks only with the 512-wide shape; surplus workgroups would
+    // only cost dispatch time on small systems.
+    const long long max_chunks = std::max(1, nblocks(nj, kTJ));
+    const long long max_iblocks = std::max(4, nblocks(nj, 512) + 1);
+    const int gf = int(std::min<long long>(kStepGrid, max_chunks * max_iblocks));
+    const int gr = std::min(kStepGrid, std::max(1, nblocks(nj, 64)));
+    if (prec == kF64) {
+        hipLaunchKernelGGL((kd_force<false>), dim3(gf), dim3(kWG), 0, st, fp, c);
+        hipLaunchKernelGGL((kd_reduce<false>), dim3(gr), dim3(256), 0, st, part, c, nj, b.cap,
+                           s.pid, b.out);
+    } else {
+        hipLaunchKernelGGL((kd_force<true>), dim3(gf), dim3(kWG), 0, st, fp, c);
+        hipLaunchKernelGGL((kd_reduce<true>), dim3(gr), dim3(256), 0, st, part, c, nj, b.cap,
+                           s.pid, b.out);
+    }
     return hipGetLastError();
 }
 

@@ -28,6 +28,8 @@
 #include <string>
 #include <vector>
 
+#include "h4_rules.h"
+
 #define H4_API extern "C" __attribute__((visibility("default")))
 
 namespace {
@@ -103,22 +105,7 @@ bool load_table(G6Table &t, const char *path, std::string &err)
     return true;
 }
 
-// ph4 host predictor for i-particles (idata.cc:341-360)
-inline void predict_ph4(const double *x, const double *v, const double *a, const double *j,
-                        double dt, double *xp, double *vp)
-{
-    if (dt == 0.0) {
-        for (int k = 0; k < 3; k++) {
-            xp[k] = x[k];
-            vp[k] = v[k];
-        }
-        return;
-    }
-    for (int k = 0; k < 3; k++) {
-        xp[k] = x[k] + dt * (v[k] + 0.5 * dt * (a[k] + dt * j[k] / 3));
-        vp[k] = v[k] + dt * (a[k] + 0.5 * dt * j[k]);
-    }
-}
+using h4::predict_ph4;  // idata.cc:341-360
 
 int load_j(Sim &s, int j)
 {
@@ -174,24 +161,9 @@ int forces(Sim &s, double t, const int *list, int ni, bool want_pot_only)
 // jdata::set_initial_timestep (jdata.cc:503-564) with the median limit
 void initial_steps(Sim &s)
 {
-    for (int j = 0; j < s.n; j++) {
-        double a2 = 0, j2 = 0;
-        for (int k = 0; k < 3; k++) {
-            a2 += s.acc[3 * j + k] * s.acc[3 * j + k];
-            j2 += s.jerk[3 * j + k] * s.jerk[3 * j + k];
-        }
-        double first;
-        if (s.eta == 0.0) first = s.dt_limit;
-        else if (a2 == 0.0 || j2 == 0.0) first = s.dt_fac * s.eta;
-        else first = s.dt_fac * s.eta * std::sqrt(a2 / j2);
-        if (std::isnan(first)) first = s.dt_fac * s.eta;
-        int e;
-        first /= 2 * std::frexp(first, &e);
-        for (int it = 0; it < 200 && first > 0 && std::fmod(s.system_time, first) != 0; it++)
-            first /= 2;
-        while (first > s.dt_limit) first /= 2;
-        s.step[j] = first;
-    }
+    for (int j = 0; j < s.n; j++)
+        s.step[j] = h4::first_step_ph4(&s.acc[3 * j], &s.jerk[3 * j], s.eta, s.dt_fac,
+                                       s.dt_limit, s.system_time);
     if (s.dt_limitm > 0 && s.n > 0) {
         std::vector<double> tmp(s.step);
         std::sort(tmp.begin(), tmp.end());
@@ -202,45 +174,11 @@ void initial_steps(Sim &s)
 }
 
 // idata::correct (idata.cc:431-512)
-constexpr double kStepFloor = 0x1p-40;
-
-void correct(Sim &s, int q, int j, double tnext, const double *opos, const double *ovel,
-             const double *oacc, const double *ojerk)
+void correct(Sim &s, int q, int j, double tnext, const double *oacc, const double *ojerk)
 {
-    (void)opos;
-    (void)ovel;
-    const double dt = tnext - s.time[j];
-    const double dt2 = dt * dt;
-    double a2 = 0, j2 = 0, k2 = 0, l2 = 0;
-    double *ip = &s.ipos[3 * q], *iv = &s.ivel[3 * q];
-    const double *ia = &s.iacc[3 * q], *ij = &s.ijerk[3 * q];
-    for (int k = 0; k < 3; k++) {
-        const double alpha = -3 * (oacc[k] - ia[k]) - dt * (2 * ojerk[k] + ij[k]);
-        const double beta = 2 * (oacc[k] - ia[k]) + dt * (ojerk[k] + ij[k]);
-        ip[k] += (alpha / 12 + beta / 20) * dt2;
-        iv[k] += (alpha / 3 + beta / 4) * dt;
-        a2 += ia[k] * ia[k];
-        j2 += (dt * ij[k]) * (dt * ij[k]);
-        k2 += (2 * alpha) * (2 * alpha);
-        l2 += (6 * beta) * (6 * beta);
-    }
-    double newstep = s.eta * dt * std::sqrt((std::sqrt(a2 * k2) + j2) / (std::sqrt(j2 * l2) + k2));
-    int e;
-    double old2 = s.step[j] / (2 * std::frexp(s.step[j], &e));
-    // bounded: tnext is a sum of powers of two no smaller than the steps
-    for (int it = 0; it < 200 && old2 > 0 && std::fmod(tnext, old2) != 0; it++) old2 /= 2;
-    if (!(newstep > 0)) newstep = old2;  // degenerate derivative sums
-    if (newstep < old2) {
-        newstep = old2 / 2;
-    } else {
-        const double t2 = 2 * old2;
-        if (newstep >= t2 && std::fmod(tnext, t2) == 0) newstep = t2;
-        else newstep = old2;
-    }
-    // Floor (never reached by FP64 forces): with force noise above the
-    // third difference of the forces the rule keeps halving; the floor keeps
-    // block times advancing and commensurate (all steps are powers of two).
-    if (newstep < kStepFloor) newstep = kStepFloor;
+    const double newstep = h4::correct_ph4(tnext, s.time[j], s.step[j], s.eta, oacc, ojerk,
+                                           &s.iacc[3 * q], &s.ijerk[3 * q], &s.ipos[3 * q],
+                                           &s.ivel[3 * q]);
     s.time[j] = tnext;
     s.step[j] = newstep;
 }
@@ -262,7 +200,7 @@ int block_step(Sim &s)
             oacc[k] = s.acc[3 * j + k];
             ojerk[k] = s.jerk[3 * j + k];
         }
-        correct(s, q, j, tnext, &s.pos[3 * j], &s.vel[3 * j], oacc, ojerk);
+        correct(s, q, j, tnext, oacc, ojerk);
         for (int k = 0; k < 3; k++) {
             s.pos[3 * j + k] = s.ipos[3 * q + k];
             s.vel[3 * j + k] = s.ivel[3 * q + k];

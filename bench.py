@@ -213,14 +213,15 @@ def tile_sweep(args) -> int:
 
 
 def evolve_blocks(m, pos, vel, eps2, eta, nblocks, precision):
-    """ph4-style run through the GRAPE-6 ABI: commit (full-N initial force)
-    then nblocks block steps with forces in `precision`; energies from FP64
-    passes.  Returns (dE/E, pairs of the block steps, seconds of the block
+    """ph4-style run through the GRAPE-6 ABI: commit (full-N initial force,
+    FP64) then nblocks block steps with forces in `precision`; energies from
+    FP64 passes.  Returns (dE/E, pairs of the block steps, seconds of the block
     steps, block and particle step counts)."""
     from amuse_amd.hermite import Hermite4
-    h = Hermite4(precision=precision)
+    # the initial forces and steps in FP64 for both paths, so that both
+    # start from the same block structure
+    h = Hermite4(precision=0)
     h.commit(m, pos, vel, eps2=eps2, eta=eta)
-    h.set_precision(0)
     e0 = sum(h.energy())
     h.set_precision(precision)
     c0 = h.counters()
@@ -240,7 +241,8 @@ def evolve_blocks(m, pos, vel, eps2, eta, nblocks, precision):
 
 def ph4_c0(args) -> int:
     """Configuration c0: Plummer N = 1024, eps2 = 1e-4, eta = 0.01 to t = 10
-    with the block-step driver on libg6.so in FP64, energy error every 0.5
+    with the block-step driver on libg6.so in FP64 (host-side driver over the
+    GRAPE-6 ABI, and the device-resident driver), energy error every 0.5
     (FP64 passes) beside the reference ph4 integrator's run on the CPU oracle
     (tests/golden/ph4_c0_energy.npz).  Mixed forces: at eta = 0.01 with a
     block budget of 4x the FP64 run's (the step rule's third force difference
@@ -256,8 +258,12 @@ def ph4_c0(args) -> int:
                "dE_over_E": ((gold["E"] - gold["E"][0]) / abs(gold["E"][0])).tolist(),
                "block_steps": gold["blocks"].tolist(), "particle_steps": gold["steps"].tolist()}}
 
-    def one(prec, eta, budget_per_out=None):
-        h = Hermite4(precision=prec)
+    def one(prec, eta, budget_per_out=None, device=False):
+        if device:
+            from amuse_amd.blocks import DeviceHermite4
+            h = DeviceHermite4(precision=prec)
+        else:
+            h = Hermite4(precision=prec)
         h.commit(m, x, v, eps2=float(gold["eps2"]), eta=eta)
         E, B, S, T = [], [], [], []
         t0 = time.perf_counter()
@@ -270,19 +276,21 @@ def ph4_c0(args) -> int:
             E.append(sum(h.energy()))
             c = h.counters()
             B.append(c["block_steps"]); S.append(c["particle_steps"]); T.append(h.time)
-            log(f"  prec={prec} eta={eta} t={t:.2f} dE/E={(E[-1] - E[0]) / abs(E[0]):.3e} "
-                f"blocks={B[-1]} reached={reached}")
+            log(f"  {'device' if device else 'host'} driver prec={prec} eta={eta} t={t:.2f} "
+                f"dE/E={(E[-1] - E[0]) / abs(E[0]):.3e} blocks={B[-1]} reached={reached}")
             if not reached:
                 break
         wall = time.perf_counter() - t0
         h.close()
         E = np.array(E)
-        return {"eta": eta, "dE_over_E": ((E - E[0]) / abs(E[0])).tolist(), "block_steps": B,
+        return {"driver": "device" if device else "host", "eta": eta,
+                "dE_over_E": ((E - E[0]) / abs(E[0])).tolist(), "block_steps": B,
                 "particle_steps": S, "reached_all": reached, "time_reached": T[-1],
                 "wall_s": wall, "gpairs_per_s": float(S[-1]) * m.size / wall / 1e9}
 
     eta0 = float(gold["eta"])
     out["fp64"] = one(0, eta0)
+    out["fp64_device_driver"] = one(0, eta0, device=True)
     budget = 4 * np.diff(np.concatenate([[0], gold["blocks"]]))
     out["mixed"] = one(1, eta0, budget)
     out["fp64_eta0.14"] = one(0, 0.14)
@@ -488,8 +496,9 @@ def main() -> int:
     Umx = 0.5 * float((m * pmx).sum())
     K = 0.5 * float((m * (vel * vel).sum(1)).sum())
 
-    cpu = None
+    cpu = cpu_best = None
     if not args.no_cpu:
+        import oracle
         threads = args.cpu_threads or min(16, os.cpu_count() or 1)
         log(f"bench: CPU baseline (g6lib restatement, {threads} threads)")
         sec, ni_cpu, cres = cpu_baseline(m, pos, vel, eps2, threads, args.cpu_per_thread)
@@ -502,6 +511,17 @@ def main() -> int:
                          f"pow() triple, per-call predictor, {args.cpu_per_thread} i per call), "
                          f"{threads} independent units on {threads} threads, {sec:.1f} s",
                "max_rel_acc_vs_gpu_fp64": float(dc.max())}
+        # best-effort CPU row: ph4's own CPU pair loop, SIMD over j, threads over i
+        ni_b = 128 * threads
+        log(f"bench: best-effort CPU loop ({threads} threads, {ni_b} i)")
+        bsec, ba, _, _, _ = oracle.cpu_best_run(threads, pos[:ni_b], vel[:ni_b], eps2, m, pos, vel)
+        db = np.linalg.norm(ba - a64[:ni_b], axis=1) / np.linalg.norm(a64[:ni_b], axis=1)
+        cpu_best = {"value": ni_b * float(n) / bsec / 1e9, "unit": "G pair-interactions/s",
+                    "cores": threads, "kind": "port",
+                    "sample": f"{ni_b} i-particles x {n} j, ph4's CPU pair loop "
+                              f"(src/amuse_ph4/src/idata.cc:195-236) in FP64, AVX2 SIMD over j, "
+                              f"OpenMP over i, {bsec:.1f} s",
+                    "max_rel_acc_vs_gpu_fp64": float(db.max())}
 
     g6.close()
     # c4's 8 block steps after the full-N evaluation, both paths, energies
@@ -544,6 +564,7 @@ def main() -> int:
                      "traffic": traffic, "kernel_ms": kmx,
                      "flop_per_pair": FLOP_PER_PAIR, "pairs_per_launch": pairs_per_launch_mx},
         "cpu_baseline": cpu,
+        "cpu_best_effort": cpu_best,
         "block_steps_8": evo,
         "energy_error": {"fp64": evo["fp64"]["dE_over_E"], "mixed": evo["mixed"]["dE_over_E"],
                          "note": "relative energy change over 8 ph4 block steps after the "
@@ -552,6 +573,8 @@ def main() -> int:
     }
     if cpu:
         out["speedup_vs_cpu"] = {"mixed": gpmx / cpu["value"], "fp64": gp64 / cpu["value"]}
+        out["speedup_vs_cpu_best_effort"] = {"mixed": gpmx / cpu_best["value"],
+                                             "fp64": gp64 / cpu_best["value"]}
     print(json.dumps(out))
     return 0
 

@@ -139,6 +139,40 @@ int g6a_adjoint_f64(int n, const double *x, const double *v, const double *m,
                     const double *ga, const double *gj, double eps2,
                     double *gx, double *gv, double *gm, void *stream);
 
+/* ---- Device-resident block-step Hermite integrator (g6_blocks.hip): the
+ * integration loop of ph4's GPU path (scheduler, i-predictor, force call,
+ * Hermite corrector with the Aarseth step; src/amuse_ph4/src/jdata.cc:
+ * 752-793, idata.cc:330-514, scheduler.cc:66-140) run on the GPU with no
+ * host round trip per block step.  The state stays on the device.
+ *
+ * g6d_create loads n particles (ids may be NULL: 1..n; m [n], x and v
+ * [n][3]), computes the initial forces and steps (ph4 defaults: dt_fac
+ * 0.0625, dt_limit 0.03125, dt_limitm 8) and returns a handle (NULL on
+ * failure).  ph4_initial_load = 1 keeps acc = jerk = 0 in the j-set until
+ * a particle's first step, as ph4 does.
+ * g6d_run advances block steps while the next block time is <= t_end, or
+ * for max_blocks (> 0) block steps, replaying `steps_per_graph` captured
+ * steps at a time; returns the block steps taken or a negative code, and
+ * sets *reached (if not NULL) to 1 when it stopped at t_end, 0 when the
+ * block budget ran out first.
+ * g6d_energy: kinetic and potential energy at the system time (FP64 pass).
+ * g6d_state: x and v predicted to the system time.  g6d_steps: per-particle
+ * time and step.  g6d_counters: block steps, particle steps, system time. */
+void *g6d_create(int n, const int ids[], const double m[], const double x[][3],
+                 const double v[][3], double eps2, double eta, double dt_fac,
+                 double dt_limit, double dt_limitm, int precision,
+                 int ph4_initial_load, int device);
+long long g6d_run(void *drv, double t_end, long long max_blocks, int steps_per_graph,
+                  int *reached);
+int g6d_energy(void *drv, double *kinetic, double *potential);
+int g6d_state(void *drv, double x[][3], double v[][3]);
+int g6d_steps(void *drv, double time[], double step[]);
+int g6d_counters(void *drv, long long *block_steps, long long *particle_steps,
+                 double *system_time);
+int g6d_set_precision(void *drv, int precision);
+const char *g6d_last_error(void *drv);
+void g6d_free(void *drv);
+
 #ifdef __cplusplus
 }
 #endif

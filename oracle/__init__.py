@@ -5,6 +5,8 @@ leg, always as the checker or as the timed CPU baseline, never as the thing
 measured or shipped.  The product library never imports this package.
 
     liboracle.so       own C restatement of lib/g6lib/g6lib.c (g6_oracle.c)
+    libcpubest.so      ph4's CPU pair loop (idata.cc:195-236) vectorised and
+                       threaded: the best-effort CPU row of the benchmark
     _ref/libg6ref.so   the reference lib/g6lib/g6lib.c compiled from its
                        source by oracle/Makefile (this container; travels to
                        the GPU box as a built file, never as source)
@@ -22,6 +24,7 @@ import numpy as np
 HERE = os.path.dirname(os.path.abspath(__file__))
 ORACLE_SO = os.path.join(HERE, "liboracle.so")
 REF_SO = os.path.join(HERE, "_ref", "libg6ref.so")
+CPUBEST_SO = os.path.join(HERE, "libcpubest.so")
 
 _D = C.POINTER(C.c_double)
 _I = C.POINTER(C.c_int)
@@ -145,6 +148,25 @@ def baseline_run(nthreads, npipes, nj, ni, ti, eps2, ids, tj, m, x, v, a2, j6, k
     sec = lib().orc_baseline_run(int(nthreads), int(npipes), int(bool(compat)), int(nj), int(ni),
                                  float(ti), float(eps2), _i(ids), _d(tj), _d(m), _d(x), _d(v),
                                  _d(a2), _d(j6), _d(k18), _d(acc), _d(jerk), _d(pot), _i(nn))
+    return sec, acc, jerk, pot, nn
+
+
+def cpu_best_run(nthreads, ix, iv, eps2, jm, jx, jv):
+    """Best-effort CPU forces (ph4's idata.cc:195-236 loop, OpenMP over i,
+    SIMD over j).  Returns (seconds, acc, jerk, pot, nn slot)."""
+    if not os.path.exists(CPUBEST_SO):
+        build()
+    L = C.CDLL(CPUBEST_SO)
+    L.cpu_best_run.restype = C.c_double
+    L.cpu_best_run.argtypes = [C.c_int, C.c_int, C.c_int, C.c_double, _D, _D, _D, _D, _D, _D,
+                               _D, _D, _I]
+    ix, iv = _f64(ix).reshape(-1, 3), _f64(iv).reshape(-1, 3)
+    jx, jv, jm = _f64(jx).reshape(-1, 3), _f64(jv).reshape(-1, 3), _f64(jm)
+    ni, nj = ix.shape[0], jm.size
+    acc = np.zeros((ni, 3)); jerk = np.zeros((ni, 3)); pot = np.zeros(ni)
+    nn = np.zeros(ni, dtype=np.int32)
+    sec = L.cpu_best_run(int(nthreads), ni, nj, float(eps2), _d(ix), _d(iv), _d(jm), _d(jx),
+                         _d(jv), _d(acc), _d(jerk), _d(pot), _i(nn))
     return sec, acc, jerk, pot, nn
 
 
