@@ -19,9 +19,12 @@
 //
 // Arrays are [n][3] row-major (torch's contiguous (n, 3)); T is float or
 // double.  Each workgroup covers 256/NJS i-particles, NJS lanes share one
-// i and split the j-range; j-tiles go through LDS.  Sums are compensated per
+// i and split the j-range (NJS chosen for ~8 workgroups per CU); j-tiles go
+// through LDS as rows of four values.  Sums are compensated per
 // tile.  No allocation: the caller passes every buffer and its stream.
 #include <hip/hip_runtime.h>
+
+#include <type_traits>
 
 #define G6A_API extern "C" __attribute__((visibility("default")))
 
@@ -46,6 +49,19 @@ __device__ __forceinline__ double rsq<double>(double s)
     return __builtin_fma(y * e, __builtin_fma(e, 0.375, 0.5), y);
 }
 
+// One LDS row of four values (ds_read_b128 for float; two for double).
+template <class T>
+struct Row4 {
+    using type = typename std::conditional<sizeof(T) == 4, float4, double4>::type;
+};
+template <class T>
+__device__ __forceinline__ typename Row4<T>::type row4(T a, T b, T c, T d)
+{
+    typename Row4<T>::type r;
+    r.x = a; r.y = b; r.z = c; r.w = d;
+    return r;
+}
+
 template <class T>
 __device__ __forceinline__ void two_sum(T &hi, T &lo, T x)
 {
@@ -61,7 +77,8 @@ __global__ void __launch_bounds__(kWG) k_forces(int n, const T *__restrict__ x,
                                                 const T *__restrict__ m, T eps2,
                                                 T *__restrict__ acc, T *__restrict__ jerk)
 {
-    __shared__ T sx[kTJ][3], sv[kTJ][3], sm[kTJ];
+    using R = typename Row4<T>::type;
+    __shared__ R sa[kTJ], sb[kTJ];  // (x y z m), (vx vy vz -)
     const int t = threadIdx.x;
     const int js = t % NJS, il = t / NJS;
     const int i = blockIdx.x * (kWG / NJS) + il;
@@ -74,22 +91,19 @@ __global__ void __launch_bounds__(kWG) k_forces(int n, const T *__restrict__ x,
         __syncthreads();
         if (t < jn) {
             const int q = tb + t;
-#pragma unroll
-            for (int k = 0; k < 3; k++) {
-                sx[t][k] = x[3 * q + k];
-                sv[t][k] = v[3 * q + k];
-            }
-            sm[t] = m[q];
+            sa[t] = row4<T>(x[3 * q], x[3 * q + 1], x[3 * q + 2], m[q]);
+            sb[t] = row4<T>(v[3 * q], v[3 * q + 1], v[3 * q + 2], T(0));
         }
         __syncthreads();
         T s6[6] = {0, 0, 0, 0, 0, 0};
         for (int jj = js; jj < jn; jj += NJS) {
-            const T dx = sx[jj][0] - xi, dy = sx[jj][1] - yi, dz = sx[jj][2] - zi;
+            const R A = sa[jj], B = sb[jj];
+            const T dx = A.x - xi, dy = A.y - yi, dz = A.z - zi;
             const T s = dx * dx + dy * dy + dz * dz + eps2;
             T y = rsq<T>(s);
             y = (tb + jj == ic) ? T(0) : y;
-            const T y2 = y * y, mi3 = sm[jj] * y * y2;
-            const T dvx = sv[jj][0] - vxi, dvy = sv[jj][1] - vyi, dvz = sv[jj][2] - vzi;
+            const T y2 = y * y, mi3 = A.w * y * y2;
+            const T dvx = B.x - vxi, dvy = B.y - vyi, dvz = B.z - vzi;
             const T rv = dx * dvx + dy * dvy + dz * dvz;
             const T c = T(-3) * rv * y2;
             s6[0] += mi3 * dx; s6[1] += mi3 * dy; s6[2] += mi3 * dz;
@@ -124,7 +138,9 @@ __global__ void __launch_bounds__(kWG) k_adjoint(int n, const T *__restrict__ x,
                                                  T *__restrict__ gx, T *__restrict__ gv,
                                                  T *__restrict__ gm)
 {
-    __shared__ T sx[kTJ][3], sv[kTJ][3], sga[kTJ][3], sgj[kTJ][3], sm[kTJ];
+    using R = typename Row4<T>::type;
+    // (x y z m), (vx vy vz -), (gA), (gJ) of each j
+    __shared__ R sa[kTJ], sb[kTJ], sc[kTJ], sd[kTJ];
     const int t = threadIdx.x;
     const int js = t % NJS, il = t / NJS;
     const int k0 = blockIdx.x * (kWG / NJS) + il;
@@ -146,22 +162,19 @@ __global__ void __launch_bounds__(kWG) k_adjoint(int n, const T *__restrict__ x,
         __syncthreads();
         if (t < jn) {
             const int p = tb + t;
-#pragma unroll
-            for (int q = 0; q < 3; q++) {
-                sx[t][q] = x[3 * p + q];
-                sv[t][q] = v[3 * p + q];
-                sga[t][q] = ga[3 * p + q];
-                sgj[t][q] = gj[3 * p + q];
-            }
-            sm[t] = m[p];
+            sa[t] = row4<T>(x[3 * p], x[3 * p + 1], x[3 * p + 2], m[p]);
+            sb[t] = row4<T>(v[3 * p], v[3 * p + 1], v[3 * p + 2], T(0));
+            sc[t] = row4<T>(ga[3 * p], ga[3 * p + 1], ga[3 * p + 2], T(0));
+            sd[t] = row4<T>(gj[3 * p], gj[3 * p + 1], gj[3 * p + 2], T(0));
         }
         __syncthreads();
         T s[13];
 #pragma unroll
         for (int q = 0; q < 13; q++) s[q] = 0;
         for (int jj = js; jj < jn; jj += NJS) {
-            const T r0 = sx[jj][0] - X[0], r1 = sx[jj][1] - X[1], r2 = sx[jj][2] - X[2];
-            const T w0 = sv[jj][0] - V[0], w1 = sv[jj][1] - V[1], w2 = sv[jj][2] - V[2];
+            const R A = sa[jj], B = sb[jj], Cg = sc[jj], Dg = sd[jj];
+            const T r0 = A.x - X[0], r1 = A.y - X[1], r2 = A.z - X[2];
+            const T w0 = B.x - V[0], w1 = B.y - V[1], w2 = B.z - V[2];
             const T ss = r0 * r0 + r1 * r1 + r2 * r2 + eps2;
             T y = rsq<T>(ss);
             y = (tb + jj == kc) ? T(0) : y;
@@ -169,8 +182,8 @@ __global__ void __launch_bounds__(kWG) k_adjoint(int n, const T *__restrict__ x,
             const T rv = r0 * w0 + r1 * w1 + r2 * w2;
             // p's gradients, weight m_k applied at the end
             {
-                const T a0 = sga[jj][0], a1 = sga[jj][1], a2 = sga[jj][2];
-                const T b0 = sgj[jj][0], b1 = sgj[jj][1], b2 = sgj[jj][2];
+                const T a0 = Cg.x, a1 = Cg.y, a2 = Cg.z;
+                const T b0 = Dg.x, b1 = Dg.y, b2 = Dg.z;
                 const T rga = r0 * a0 + r1 * a1 + r2 * a2;
                 const T vgj = w0 * b0 + w1 * b1 + w2 * b2;
                 const T rgj = r0 * b0 + r1 * b1 + r2 * b2;
@@ -188,7 +201,7 @@ __global__ void __launch_bounds__(kWG) k_adjoint(int n, const T *__restrict__ x,
             }
             // k's own gradients, weight m_p
             {
-                const T mp = sm[jj];
+                const T mp = A.w;
                 const T my3 = mp * y3, my5 = mp * y5, my7 = mp * y7;
                 const T rga = r0 * GA[0] + r1 * GA[1] + r2 * GA[2];
                 const T vgj = w0 * GJ[0] + w1 * GJ[1] + w2 * GJ[2];
@@ -226,17 +239,45 @@ __global__ void __launch_bounds__(kWG) k_adjoint(int n, const T *__restrict__ x,
     }
 }
 
+// Lanes per i: the smallest power of two from 8 that gives ~2048
+// workgroups (8 per CU) for n particles, at most 64 (one wave per i).
+inline int lanes_per_i(int n)
+{
+    int njs = 8;
+    while (njs < 64 && (long long)n * njs < 2048ll * kWG) njs *= 2;
+    return njs;
+}
+
+template <class T, int NJS>
+void launch_forces_t(int n, const T *x, const T *v, const T *m, T eps2, T *acc, T *jerk,
+                     hipStream_t st)
+{
+    const int grid = (n + kWG / NJS - 1) / (kWG / NJS);
+    hipLaunchKernelGGL((k_forces<T, NJS>), dim3(grid), dim3(kWG), 0, st, n, x, v, m, eps2, acc,
+                       jerk);
+}
+
 template <class T>
 int launch_forces(int n, const T *x, const T *v, const T *m, T eps2, T *acc, T *jerk,
                   hipStream_t st)
 {
     if (n <= 0) return 0;
-    // enough workgroups for 256 CUs: split j over lanes when n is small
-    constexpr int NJS = 8;
-    const int grid = (n + kWG / NJS - 1) / (kWG / NJS);
-    hipLaunchKernelGGL((k_forces<T, NJS>), dim3(grid), dim3(kWG), 0, st, n, x, v, m, eps2, acc,
-                       jerk);
+    switch (lanes_per_i(n)) {
+    case 8: launch_forces_t<T, 8>(n, x, v, m, eps2, acc, jerk, st); break;
+    case 16: launch_forces_t<T, 16>(n, x, v, m, eps2, acc, jerk, st); break;
+    case 32: launch_forces_t<T, 32>(n, x, v, m, eps2, acc, jerk, st); break;
+    default: launch_forces_t<T, 64>(n, x, v, m, eps2, acc, jerk, st); break;
+    }
     return hipGetLastError() == hipSuccess ? 0 : -5;
+}
+
+template <class T, int NJS>
+void launch_adjoint_t(int n, const T *x, const T *v, const T *m, const T *ga, const T *gj,
+                      T eps2, T *gx, T *gv, T *gm, hipStream_t st)
+{
+    const int grid = (n + kWG / NJS - 1) / (kWG / NJS);
+    hipLaunchKernelGGL((k_adjoint<T, NJS>), dim3(grid), dim3(kWG), 0, st, n, x, v, m, ga, gj, eps2,
+                       gx, gv, gm);
 }
 
 template <class T>
@@ -244,10 +285,12 @@ int launch_adjoint(int n, const T *x, const T *v, const T *m, const T *ga, const
                    T *gx, T *gv, T *gm, hipStream_t st)
 {
     if (n <= 0) return 0;
-    constexpr int NJS = 8;
-    const int grid = (n + kWG / NJS - 1) / (kWG / NJS);
-    hipLaunchKernelGGL((k_adjoint<T, NJS>), dim3(grid), dim3(kWG), 0, st, n, x, v, m, ga, gj, eps2,
-                       gx, gv, gm);
+    switch (lanes_per_i(n)) {
+    case 8: launch_adjoint_t<T, 8>(n, x, v, m, ga, gj, eps2, gx, gv, gm, st); break;
+    case 16: launch_adjoint_t<T, 16>(n, x, v, m, ga, gj, eps2, gx, gv, gm, st); break;
+    case 32: launch_adjoint_t<T, 32>(n, x, v, m, ga, gj, eps2, gx, gv, gm, st); break;
+    default: launch_adjoint_t<T, 64>(n, x, v, m, ga, gj, eps2, gx, gv, gm, st); break;
+    }
     return hipGetLastError() == hipSuccess ? 0 : -5;
 }
 

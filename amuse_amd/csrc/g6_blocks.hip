@@ -20,14 +20,16 @@
 // limit (jdata.cc:503-564) computed on the host from the downloaded
 // acc/jerk, the same arithmetic as the host driver (h4_rules.h).
 //
-// One step is seven kernels (begin, next-time reduction, selection with the
-// i-prediction, j-prediction, force, reduce, correction); g6d_run captures
+// One step is five kernels (next-time reduction, selection with the i- and
+// j-prediction, force, reduce, correction; the control-block protocol is
+// described with g6k::StepCtl); g6d_run captures
 // `steps_per_graph` of them into a hipGraph and replays it, reading the
 // control block (done flag, counters) back after each replay.  Kernels of a
 // finished run return at once, so a replay never overshoots t_end or the
 // block budget.
 #include "../../include/g6x.h"
 #include "g6_kernels.h"
+#include "g6_predict.h"
 #include "h4_rules.h"
 
 #include <algorithm>
@@ -81,21 +83,19 @@ inline int nblk(int n, int b) { return (n + b - 1) / b; }
 // ---------------------------------------------------------------------------
 // step kernels
 // ---------------------------------------------------------------------------
-__global__ void kb_begin(StepCtl *c)
-{
-    if (c->done) return;
-    if (c->budget == 0) {
-        c->done = 2;  // block budget spent
-        return;
-    }
-    c->tmin = 0x7ff0000000000000ull;  // +inf
-    c->ni = 0;
-}
-
+// Next block time: min over particles of t + dt (wave shuffles, then one
+// 64-bit atomicMin per wave on the order-preserving bits).  Also ends the
+// run when the block budget is spent and resets the block counter.
 __global__ void __launch_bounds__(256) kb_tmin(const double *__restrict__ t,
                                                const double *__restrict__ dt, int n, StepCtl *c)
 {
     if (c->done) return;
+    const bool first = blockIdx.x == 0 && threadIdx.x == 0;
+    if (c->budget == 0) {
+        if (first) c->done = 2;  // block budget spent
+        return;
+    }
+    if (first) c->ni = 0;
     double lo = __builtin_inf();
     for (int j = blockIdx.x * blockDim.x + threadIdx.x; j < n; j += gridDim.x * blockDim.x)
         lo = fmin(lo, t[j] + dt[j]);
@@ -107,8 +107,10 @@ __global__ void __launch_bounds__(256) kb_tmin(const double *__restrict__ t,
 
 // The block: particles with t + dt equal to the next block time, appended in
 // any order (each i's force and correction are independent of the order),
-// predicted to it into the i-block rows.
-__global__ void __launch_bounds__(256) kb_select(State s, IBlock b, StepCtl *c)
+// predicted to it into the i-block rows; every j-slot is predicted to it as
+// well (the library's predictor at set_ti).
+__global__ void __launch_bounds__(256) kb_select(State s, JStore js, IBlock b, StepCtl *c,
+                                                 int prec, int compat)
 {
     if (c->done) return;
     const double tnext = __longlong_as_double(static_cast<long long>(c->tmin));
@@ -116,7 +118,19 @@ __global__ void __launch_bounds__(256) kb_select(State s, IBlock b, StepCtl *c)
         if (blockIdx.x == 0 && threadIdx.x == 0) c->done = 1;  // t_end reached
         return;
     }
+    if (blockIdx.x == 0 && threadIdx.x == 0) c->tnext = tnext;
     for (int j = blockIdx.x * blockDim.x + threadIdx.x; j < s.n; j += gridDim.x * blockDim.x) {
+        {
+            double x[3], v[3], a2[3], j6[3];
+            const double k18[3] = {0.0, 0.0, 0.0};
+#pragma unroll
+            for (int k = 0; k < 3; k++) {
+                x[k] = js.x[k][j]; v[k] = js.v[k][j]; a2[k] = js.a2[k][j]; j6[k] = js.j6[k][j];
+            }
+            const g6k::Pred pr = g6k::predict_slot(tnext, js.tj[j], x, v, a2, j6, k18, compat,
+                                                   false);
+            g6k::write_pred(js, j, prec, js.id[j], pr);
+        }
         if (s.t[j] + s.dt[j] != tnext) continue;
         const int q = atomicAdd(&c->ni, 1);
         s.list[q] = j;
@@ -208,7 +222,7 @@ __global__ void __launch_bounds__(256) kb_correct(State s, JStore js, IBlock b, 
                                                   double eta)
 {
     if (c->done) return;
-    const double tnext = __longlong_as_double(static_cast<long long>(c->tmin));
+    const double tnext = c->tnext;
     const int ni = c->ni;
     const int S = b.cap;
     if (blockIdx.x == 0 && threadIdx.x == 0) {
@@ -250,6 +264,8 @@ __global__ void kb_arm(StepCtl *c, double t_end, long long budget)
     c->t_end = t_end;
     c->budget = budget;
     c->done = 0;
+    c->tmin = 0x7ff0000000000000ull;  // +inf
+    c->ni = 0;
 }
 
 // ---------------------------------------------------------------------------
@@ -261,10 +277,9 @@ int enqueue_step(Driver *D)
 {
     const int n = D->st.n;
     const int g = std::min(nblk(n, 256), kGrid);
-    hipLaunchKernelGGL(kb_begin, dim3(1), dim3(1), 0, D->stream, D->ctl);
     hipLaunchKernelGGL(kb_tmin, dim3(g), dim3(256), 0, D->stream, D->st.t, D->st.dt, n, D->ctl);
-    hipLaunchKernelGGL(kb_select, dim3(g), dim3(256), 0, D->stream, D->st, D->ib, D->ctl);
-    DCK(g6k::launch_step_predict(D->js, n, D->ctl, 0, D->prec, D->compat, D->stream));
+    hipLaunchKernelGGL(kb_select, dim3(g), dim3(256), 0, D->stream, D->st, D->js, D->ib, D->ctl,
+                       D->prec, D->compat);
     DCK(g6k::launch_step_force(D->js, D->ib, D->part, n, D->eps2, D->prec, D->ctl, D->stream));
     hipLaunchKernelGGL(kb_correct, dim3(g), dim3(256), 0, D->stream, D->st, D->js, D->ib, D->ctl,
                        D->eta);

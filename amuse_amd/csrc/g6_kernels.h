@@ -82,14 +82,21 @@ struct ForceCfg {
 // the predictor, force and reduce kernels of a step read it from this
 // control block and size their work themselves; one captured launch
 // sequence then serves every step.
+//
+// Protocol of one step: the next-time kernel resets ni and folds min(t + dt)
+// into tmin; the selection kernel turns tmin into tnext, appends the block to
+// ni; the force kernel resets tmin for the next step (nothing reads it after
+// the selection); the correction kernel advances the counters.  Each field
+// is reset by a kernel that no other kernel of the same launch reads it in.
 struct StepCtl {
     unsigned long long tmin;  // bits of min(t + dt) (positive doubles order as integers)
     int ni;                   // active particles of the current step
-    int done;                 // 1: t_end reached or block budget spent
+    int done;                 // 1: t_end reached, 2: block budget spent
     long long blocks, psteps; // block and particle steps taken
     long long budget;         // block steps left in the current run (< 0: no limit)
     double t_end;
     double system_time;
+    double tnext;             // block time of the current step
 };
 
 struct StepPlan {
@@ -132,15 +139,15 @@ __host__ __device__ inline StepPlan plan_step(int ni, int nj)
     return sp;
 }
 
-// Predicts every j-slot to the step's block time (at_system = 0) or to
-// c->system_time (1); no-op once the step is done.
+// Predicts every j-slot to the step's block time c->tnext (at_system = 0)
+// or to c->system_time (1); no-op once the run is done.
 hipError_t launch_step_predict(const JStore &s, int nj, const StepCtl *c, int at_system,
                                int prec, int compat, hipStream_t st);
 // Forces (acc, jerk, pot; no nn) of the step's i-block (b rows 0..ni-1,
 // ni = c->ni) against j-slots [0, nj) into b.out, with partials in part
 // (capacity kStepPartBudget * fields doubles) reduced by a second kernel.
 hipError_t launch_step_force(const JStore &s, const IBlock &b, double *part, int nj,
-                             double eps2, int prec, const StepCtl *c, hipStream_t st);
+                             double eps2, int prec, StepCtl *c, hipStream_t st);
 
 size_t jstore_bytes(int cap);
 void jstore_carve(JStore &s, void *base, int cap);

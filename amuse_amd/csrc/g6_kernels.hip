@@ -23,13 +23,13 @@
 // lexicographically, which reproduces the sequential first-wins rule for any
 // j-split.
 #include "g6_kernels.h"
+#include "g6_predict.h"
 
 #include <algorithm>
 #include <cfloat>
 
 namespace g6k {
 
-static constexpr float kFarF = 1.0e18f;  // empty-slot position, mixed path
 
 // inner-loop unroll depths (build-time tunables)
 #ifndef G6_MX_UNROLL
@@ -92,67 +92,6 @@ void iblock_carve(IBlock &b, void *base, int cap)
     b.nn = reinterpret_cast<int *>(p + 15 * row);
     b.cap = cap;
     b.base = base;
-}
-
-// ------------------------------------------------------------------------
-// Predictor (lib/g6lib/g6lib.c:64-92).  Contraction is off so each add and
-// multiply rounds once, as in the reference; the nearest-neighbour search
-// depends on predicted positions being bit-identical to it.
-// ------------------------------------------------------------------------
-struct Pred {
-    double x[3], v[3];
-};
-
-__device__ __forceinline__ Pred predict_slot(double ti, double tj,
-                                             const double x[3], const double v[3],
-                                             const double a2[3], const double j6[3],
-                                             const double k18[3], int compat, bool use_k18)
-{
-#pragma clang fp contract(off)
-    Pred r;
-    const double dt = ti - tj;
-    const double dt2 = dt * dt;
-    const double dt3 = dt2 * dt;
-    const double dt4 = dt2 * dt2;
-#pragma unroll
-    for (int k = 0; k < 3; k++) {
-        double dx = v[k] * dt;
-        dx = dx + a2[k] * dt2;
-        dx = dx + j6[k] * dt3;
-        if (use_k18) dx = dx + k18[k] * 3.0 / 4.0 * dt4;
-        r.x[k] = dx + x[k];
-        double dv = a2[k] * 2.0 * dt;
-        if (compat) {
-            dv = dv + j6[k] * 3.0 * dt3;
-            if (use_k18) dv = dv + k18[k] * 6.0 * dt4;
-        } else {
-            dv = dv + j6[k] * 3.0 * dt2;
-            if (use_k18) dv = dv + k18[k] * 3.0 * dt3;
-        }
-        r.v[k] = dv + v[k];
-    }
-    return r;
-}
-
-__device__ __forceinline__ void write_pred(const JStore &s, int j, int prec, int id, const Pred &p)
-{
-    if (prec == kF64) {
-#pragma unroll
-        for (int k = 0; k < 3; k++) {
-            s.px[k][j] = id == -1 ? kFar : p.x[k];
-            s.pv[k][j] = id == -1 ? 0.0 : p.v[k];
-        }
-    } else {
-#pragma unroll
-        for (int k = 0; k < 3; k++) {
-            const double xp = p.x[k];
-            const float hi = float(xp);
-            const float lo = float(xp - double(hi));
-            s.hx[k][j] = id == -1 ? kFarF : hi;
-            s.lx[k][j] = id == -1 ? 0.0f : lo;
-            s.fv[k][j] = id == -1 ? 0.0f : float(p.v[k]);
-        }
-    }
 }
 
 __global__ void __launch_bounds__(256) k_reset(JStore s, int lo, int hi)
@@ -1021,8 +960,7 @@ __global__ void __launch_bounds__(256) k_reduce(const double *__restrict__ part,
 // ------------------------------------------------------------------------
 __device__ __forceinline__ double step_time(const StepCtl *c, int at_system)
 {
-    return at_system ? c->system_time
-                     : __longlong_as_double(static_cast<long long>(c->tmin));
+    return at_system ? c->system_time : c->tnext;
 }
 
 __global__ void __launch_bounds__(256) kd_predict(JStore s, int nj, const StepCtl *c,
@@ -1043,9 +981,11 @@ __global__ void __launch_bounds__(256) kd_predict(JStore s, int nj, const StepCt
 }
 
 template <bool MX>
-__global__ void __launch_bounds__(kWG) kd_force(FParams p, const StepCtl *c)
+__global__ void __launch_bounds__(kWG) kd_force(FParams p, StepCtl *c)
 {
     if (c->done) return;
+    // the block is selected: the next step's minimum starts from +inf
+    if (blockIdx.x == 0 && threadIdx.x == 0) c->tmin = 0x7ff0000000000000ull;
     const int ni = c->ni;
     if (ni <= 0) return;
     const StepPlan sp = plan_step(ni, p.nj);
@@ -1316,7 +1256,7 @@ hipError_t launch_step_predict(const JStore &s, int nj, const StepCtl *c, int at
 }
 
 hipError_t launch_step_force(const JStore &s, const IBlock &b, double *part, int nj,
-                             double eps2, int prec, const StepCtl *c, hipStream_t st)
+                             double eps2, int prec, StepCtl *c, hipStream_t st)
 {
     if (nj <= 0) return hipSuccess;
     FParams fp{};

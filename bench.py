@@ -212,15 +212,17 @@ def tile_sweep(args) -> int:
     return 0
 
 
-def evolve_blocks(m, pos, vel, eps2, eta, nblocks, precision):
-    """ph4-style run through the GRAPE-6 ABI: commit (full-N initial force,
-    FP64) then nblocks block steps with forces in `precision`; energies from
-    FP64 passes.  Returns (dE/E, pairs of the block steps, seconds of the block
-    steps, block and particle step counts)."""
+def evolve_blocks(m, pos, vel, eps2, eta, nblocks, precision, device=False):
+    """ph4-style run, host-side driver through the GRAPE-6 ABI or the
+    device-resident driver: commit (full-N initial force, FP64) then nblocks
+    block steps with forces in `precision`; energies from FP64 passes.
+    Returns (dE/E, pairs of the block steps, seconds of the block steps,
+    block and particle step counts)."""
+    from amuse_amd.blocks import DeviceHermite4
     from amuse_amd.hermite import Hermite4
     # the initial forces and steps in FP64 for both paths, so that both
     # start from the same block structure
-    h = Hermite4(precision=0)
+    h = DeviceHermite4(precision=0) if device else Hermite4(precision=0)
     h.commit(m, pos, vel, eps2=eps2, eta=eta)
     e0 = sum(h.energy())
     h.set_precision(precision)
@@ -428,7 +430,10 @@ def main() -> int:
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--n", type=int, default=1 << 20)
     ap.add_argument("--eps2", type=float, default=1e-8)
-    ap.add_argument("--eta", type=float, default=0.01, help="block steps: Aarseth eta")
+    ap.add_argument("--eta", type=float, default=0.14,
+                    help="c4 block steps: Aarseth eta (ph4's default timestep_parameter, "
+                         "src/amuse_ph4/interface.py:514-520; at 0.01 FP32 force rounding "
+                         "exceeds the step rule's third force difference)")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-threads", type=int, default=0)
     ap.add_argument("--cpu-per-thread", type=int, default=256)
@@ -528,7 +533,9 @@ def main() -> int:
     # from FP64 passes (the ph4-style driver over the same library)
     log("bench: 8 block steps, FP64 then mixed forces")
     evo = {"fp64": evolve_blocks(m, pos, vel, eps2, args.eta, 8, 0),
-           "mixed": evolve_blocks(m, pos, vel, eps2, args.eta, 8, 1)}
+           "mixed": evolve_blocks(m, pos, vel, eps2, args.eta, 8, 1),
+           "fp64_device_driver": evolve_blocks(m, pos, vel, eps2, args.eta, 8, 0, True),
+           "mixed_device_driver": evolve_blocks(m, pos, vel, eps2, args.eta, 8, 1, True)}
 
     traffic = load_traffic("mixed_full_n")
     out = {
@@ -567,9 +574,13 @@ def main() -> int:
         "cpu_best_effort": cpu_best,
         "block_steps_8": evo,
         "energy_error": {"fp64": evo["fp64"]["dE_over_E"], "mixed": evo["mixed"]["dE_over_E"],
+                         "eta": args.eta,
                          "note": "relative energy change over 8 ph4 block steps after the "
-                                 "full-N evaluation, forces in the named path, energies "
-                                 "from FP64 passes"},
+                                 "full-N evaluation (initial forces and steps in FP64 for "
+                                 "both), step forces in the named path, energies from FP64 "
+                                 "passes; the first blocks hold the particles with the "
+                                 "smallest steps, so the change is near the FP64 resolution "
+                                 "of the total energy"},
     }
     if cpu:
         out["speedup_vs_cpu"] = {"mixed": gpmx / cpu["value"], "fp64": gp64 / cpu["value"]}
