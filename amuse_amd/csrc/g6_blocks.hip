@@ -491,6 +491,7 @@ G6D_API void *g6d_create(int n, const int *ids, const double *m, const double (*
 G6D_API long long g6d_run(void *h, double t_end, long long max_blocks, int steps_per_graph,
                           int *reached)
 {
+    if (!h) return G6X_ERR_ARG;
     Driver *D = static_cast<Driver *>(h);
     if (steps_per_graph < 1) steps_per_graph = 1;
     if (!D->graph || D->graph_steps != steps_per_graph || D->graph_prec != D->prec)
@@ -521,6 +522,7 @@ G6D_API long long g6d_run(void *h, double t_end, long long max_blocks, int steps
 // every particle predicted to it, summed on the host in particle order).
 G6D_API int g6d_energy(void *h, double *kin, double *pot)
 {
+    if (!h || !kin || !pot) return G6X_ERR_ARG;
     Driver *D = static_cast<Driver *>(h);
     g6b::State &s = D->st;
     const int n = s.n;
@@ -552,6 +554,7 @@ G6D_API int g6d_energy(void *h, double *kin, double *pot)
 // Positions and velocities predicted to the system time ([n][3] each).
 G6D_API int g6d_state(void *h, double (*x)[3], double (*v)[3])
 {
+    if (!h || !x || !v) return G6X_ERR_ARG;
     Driver *D = static_cast<Driver *>(h);
     g6b::State &s = D->st;
     const int n = s.n;
@@ -584,6 +587,7 @@ G6D_API int g6d_state(void *h, double (*x)[3], double (*v)[3])
 // Per-particle times and steps.
 G6D_API int g6d_steps(void *h, double *time, double *step)
 {
+    if (!h || !time || !step) return G6X_ERR_ARG;
     Driver *D = static_cast<Driver *>(h);
     const size_t nd = size_t(D->st.n);
     if (hipMemcpyAsync(time, D->st.t, nd * 8, hipMemcpyDeviceToHost, D->stream) != hipSuccess ||
@@ -595,6 +599,7 @@ G6D_API int g6d_steps(void *h, double *time, double *step)
 
 G6D_API int g6d_counters(void *h, long long *blocks, long long *psteps, double *system_time)
 {
+    if (!h || !blocks || !psteps || !system_time) return G6X_ERR_ARG;
     Driver *D = static_cast<Driver *>(h);
     if (g6b::read_ctl(D)) return G6X_ERR_HIP;
     *blocks = D->h_ctl->blocks;
@@ -605,13 +610,17 @@ G6D_API int g6d_counters(void *h, long long *blocks, long long *psteps, double *
 
 G6D_API int g6d_set_precision(void *h, int precision)
 {
+    if (!h) return G6X_ERR_ARG;
     Driver *D = static_cast<Driver *>(h);
     if (precision != G6X_FP64 && precision != G6X_MIXED) return G6X_ERR_ARG;
     D->prec = precision;
     return 0;
 }
 
-G6D_API const char *g6d_last_error(void *h) { return static_cast<Driver *>(h)->err.c_str(); }
+G6D_API const char *g6d_last_error(void *h)
+{
+    return h ? static_cast<Driver *>(h)->err.c_str() : "null driver handle";
+}
 
 G6D_API void g6d_free(void *h)
 {

@@ -483,8 +483,9 @@ struct IF {
 struct AccF {
     // sub-tile FP32 sums
     float ax, ay, az, jx, jy, jz, pot;
-    float best;
-    int addr;
+    float best;   // nearest-neighbour r^2 so far (tiles before the current one)
+    int addr;     // its slot
+    float tbest;  // running minimum of r^2 over the current block of j
 };
 // double-single accumulator (hi + lo), fed with TwoSum
 struct DSAcc {
@@ -500,25 +501,39 @@ __device__ __forceinline__ void two_sum_add(float &hi, float &lo, float x)
     lo += err;
 }
 
-template <bool NN>
+// Difference vector and r^2 of one pair, mixed path: dx from double-single
+// positions, r^2 = (dx dx + dy dy) + dz dz with FMAs.  The self pair gets
+// r^2 = +inf, so s = +inf, r^-1/2 = 0 and every term is exactly zero, and
+// it can never be the nearest neighbour.
+__device__ __forceinline__ float pair_r2_mx(const IF &I, float4 ja, float4 jb, float4 jc,
+                                            float &dx, float &dy, float &dz)
+{
+    dx = (ja.x - I.hx) + (jb.x - I.lx);
+    dy = (ja.y - I.hy) + (jb.y - I.ly);
+    dz = (ja.z - I.hz) + (jb.z - I.lz);
+    const float r2 = __builtin_fmaf(dz, dz, __builtin_fmaf(dy, dy, dx * dx));
+    return __float_as_int(jc.z) != I.id ? r2 : __builtin_inff();
+}
+
+// NNM: 0 no nearest neighbour; 1 per-pair compare, first slot wins; 2 only a
+// running minimum of r^2 over a block of j, the caller finding its slot by
+// a rescan of the block when the minimum improved (rare after the first
+// blocks), which takes the compare and two selects out of the pair.
+template <int NNM>
 __device__ __forceinline__ void pair_mx(const IF &I, AccF &a, float4 ja, float4 jb, float4 jc,
                                         int addr, float eps2)
 {
-    const float dx = (ja.x - I.hx) + (jb.x - I.lx);
-    const float dy = (ja.y - I.hy) + (jb.y - I.ly);
-    const float dz = (ja.z - I.hz) + (jb.z - I.lz);
-    const int idj = __float_as_int(jc.z);
-    // same s in both variants (forces independent of the nn request)
-    const float r2 = __builtin_fmaf(dz, dz, __builtin_fmaf(dy, dy, dx * dx));
+    float dx, dy, dz;
+    const float r2 = pair_r2_mx(I, ja, jb, jc, dx, dy, dz);
+    // same s in every variant (forces independent of the nn request)
     const float s = r2 + eps2;
-    const bool valid = idj != I.id;
-    float y = __builtin_amdgcn_rsqf(s);
-    y = valid ? y : 0.0f;
-    if (NN) {
-        const bool better = valid && (r2 < a.best);
+    const float y = __builtin_amdgcn_rsqf(s);
+    if (NNM == 1) {
+        const bool better = r2 < a.best;
         a.best = better ? r2 : a.best;
         a.addr = better ? addr : a.addr;
     }
+    if (NNM == 2) a.tbest = fminf(a.tbest, r2);
     const float y2 = y * y;
     const float mi = ja.w * y;
     const float mi3 = mi * y2;
@@ -578,7 +593,7 @@ __device__ __forceinline__ void force_mx_item(const FParams &p, TileF &tile, int
         I[q].hz = float(z); I[q].lz = float(z - double(I[q].hz));
         I[q].vx = float(p.ivx[i]); I[q].vy = float(p.ivy[i]); I[q].vz = float(p.ivz[i]);
         I[q].id = p.iid[i];
-        A[q] = AccF{0, 0, 0, 0, 0, 0, 0, __builtin_inff(), -1};
+        A[q] = AccF{0, 0, 0, 0, 0, 0, 0, __builtin_inff(), -1, __builtin_inff()};
 #pragma unroll
         for (int k = 0; k < 7; k++) { D[q].h[k] = 0.0f; D[q].l[k] = 0.0f; }
     }
@@ -602,10 +617,34 @@ __device__ __forceinline__ void force_mx_item(const FParams &p, TileF &tile, int
             for (int jj = jb; jj < je; jj += NJS) {
                 const float4 a = tile.a[jj], b = tile.b[jj], c = tile.c[jj];
 #pragma unroll
-                for (int q = 0; q < IPL; q++) pair_mx<NN>(I[q], A[q], a, b, c, tb + jj, eps2);
+                for (int q = 0; q < IPL; q++)
+                    pair_mx<NN ? 2 : 0>(I[q], A[q], a, b, c, tb + jj, eps2);
             }
 #pragma unroll
             for (int q = 0; q < IPL; q++) flush_mx(A[q], D[q]);
+            if (NN) {
+                // A minimum over this block of the lane's j below the best
+                // so far (strictly: an equal r^2 keeps its earlier slot):
+                // its slot is the first j of the block with that r^2.  The
+                // block is short (kFlush pairs), so the rescan stays cheap
+                // however often the minimum improves.
+#pragma unroll
+                for (int q = 0; q < IPL; q++) {
+                    const float tbest = A[q].tbest;
+                    if (tbest < A[q].best) {
+                        for (int jj = jb; jj < je; jj += NJS) {
+                            float dx, dy, dz;
+                            if (pair_r2_mx(I[q], tile.a[jj], tile.b[jj], tile.c[jj], dx, dy,
+                                           dz) == tbest) {
+                                A[q].addr = tb + jj;
+                                break;
+                            }
+                        }
+                        A[q].best = tbest;
+                    }
+                    A[q].tbest = __builtin_inff();
+                }
+            }
         }
     }
 
@@ -772,7 +811,7 @@ __global__ void __launch_bounds__(kWG) k_force_mx_s(FParams p)
         I[q].hz = float(z); I[q].lz = float(z - double(I[q].hz));
         I[q].vx = float(p.ivx[i]); I[q].vy = float(p.ivy[i]); I[q].vz = float(p.ivz[i]);
         I[q].id = p.iid[i];
-        A[q] = AccF{0, 0, 0, 0, 0, 0, 0, __builtin_inff(), -1};
+        A[q] = AccF{0, 0, 0, 0, 0, 0, 0, __builtin_inff(), -1, __builtin_inff()};
 #pragma unroll
         for (int k = 0; k < 7; k++) { D[q].h[k] = 0.0f; D[q].l[k] = 0.0f; }
     }
@@ -787,7 +826,7 @@ __global__ void __launch_bounds__(kWG) k_force_mx_s(FParams p)
             const float4 b = make_float4(p.lx[j], p.ly[j], p.lz[j], p.fvx[j]);
             const float4 c = make_float4(p.fvy[j], p.fvz[j], __int_as_float(p.jid[j]), 0.0f);
 #pragma unroll
-            for (int q = 0; q < IPL; q++) pair_mx<NN>(I[q], A[q], a, b, c, j, eps2);
+            for (int q = 0; q < IPL; q++) pair_mx<NN ? 1 : 0>(I[q], A[q], a, b, c, j, eps2);
         }
 #pragma unroll
         for (int q = 0; q < IPL; q++) flush_mx(A[q], D[q]);
@@ -1275,13 +1314,11 @@ hipError_t launch_step_force(const JStore &s, const IBlock &b, double *part, int
     fp.nn = b.nn;
     fp.ostride = b.cap;
     fp.part = part;
-    // Grids sized for the largest step of this system (ni <= nj): a plan has
-    // iblocks * nchunk <= 2048 + iblocks items, nchunk <= nj / kTJ, and more
-    // than 4 i-blocks only with the 512-wide shape; surplus workgroups would
-    // only cost dispatch time on small systems.
-    const long long max_chunks = std::max(1, nblocks(nj, kTJ));
-    const long long max_iblocks = std::max(4, nblocks(nj, 512) + 1);
-    const int gf = int(std::min<long long>(kStepGrid, max_chunks * max_iblocks));
+    // Grids sized for the largest step of this system (ni <= nj): the
+    // narrowest shape puts 4 i in a workgroup, so nj / 4 workgroups cover a
+    // whole-system step; surplus workgroups would only cost dispatch time on
+    // small systems, and larger plans loop over their items.
+    const int gf = std::min(kStepGrid, std::max(64, nblocks(nj, 4)));
     const int gr = std::min(kStepGrid, std::max(1, nblocks(nj, 64)));
     if (prec == kF64) {
         hipLaunchKernelGGL((kd_force<false>), dim3(gf), dim3(kWG), 0, st, fp, c);

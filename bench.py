@@ -301,6 +301,34 @@ def ph4_c0(args) -> int:
     return 0
 
 
+def driver_study(args) -> int:
+    """Device-resident block-step driver: Plummer N in --driver-n (default
+    1024, 16384, 65536), eps2 = 1e-4, eta = 0.01, FP64; time per block step
+    from t = 1/64 to --driver-t (default 1/16)."""
+    from amuse_amd import ic
+    from amuse_amd.blocks import DeviceHermite4
+    rows = []
+    for n in [int(q) for q in args.driver_n.split(",")]:
+        m, x, v = ic.plummer(n, 1024)
+        d = DeviceHermite4(precision=0)
+        d.commit(m, x, v, eps2=1e-4, eta=0.01)
+        d.evolve(1 / 64)
+        c0 = d.counters()
+        t0 = time.perf_counter()
+        d.evolve(args.driver_t)
+        dt = time.perf_counter() - t0
+        c1 = d.counters()
+        d.close()
+        nb = c1["block_steps"] - c0["block_steps"]
+        ps = c1["particle_steps"] - c0["particle_steps"]
+        rows.append({"n": n, "block_steps": nb, "us_per_block": dt / nb * 1e6,
+                     "mean_block": ps / nb, "gpairs_per_s": ps * n / dt / 1e9})
+        log(f"  driver n={n}: {rows[-1]}")
+    print(json.dumps({"workload": "device-resident block steps, Plummer, eps2=1e-4, eta=0.01, "
+                                  "FP64", "rows": rows}))
+    return 0
+
+
 def clusters_c2(args) -> int:
     """Configuration c2: 4096 clusters, N log-uniform in [16, 2048], Kroupa
     0.08-20 normalised to M = 1 per cluster, eps2 = 1e-6, all to t = 1 in
@@ -442,11 +470,13 @@ def main() -> int:
                     help="c2: hermite dt_param (step = dt_param x collision time)")
     ap.add_argument("--max-rounds", type=int, default=0)
     ap.add_argument("--n-c3", type=int, default=16384)
+    ap.add_argument("--driver-n", default="1024,16384,65536", help="driver: system sizes")
+    ap.add_argument("--driver-t", type=float, default=1 / 16, help="driver: end time")
     ap.add_argument("--steps-c3", type=int, default=256)
     ap.add_argument("--copy-calib", action="store_true",
                     help="only run a plain 1 GiB device copy (HBM counter calibration)")
     ap.add_argument("--workload", default="full",
-                    choices=["full", "blocks", "tiles", "ph4", "clusters", "adjoint"],
+                    choices=["full", "blocks", "tiles", "ph4", "clusters", "adjoint", "driver"],
                     help="full: full-N force pass at N=2^20 (default); blocks: i-blocks of "
                          "2^k against N=65536 with per-j times, FP64 vs mixed errors")
     args = ap.parse_args()
@@ -462,6 +492,8 @@ def main() -> int:
         return clusters_c2(args)
     if args.workload == "adjoint":
         return adjoint_c3(args)
+    if args.workload == "driver":
+        return driver_study(args)
     if args.gpus != 1:
         print("bench.py measures one GPU: the force path is built and measured on a single "
               "MI355X (multi-GPU j-domain sharding is a design note only)", file=sys.stderr)

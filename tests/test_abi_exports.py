@@ -78,3 +78,25 @@ def test_calls_before_open_fail_cleanly_without_gpu():
     assert lib.g6x_set_precision(7) == -3
     assert lib.g6x_set_tiles(3, 1, 0, 0) == -3
     assert lib.g6x_set_tiles(2, 1, 0, 1) == -3
+
+
+def test_device_driver_rejects_bad_handles_without_gpu():
+    """g6d_* entry points check their handle and arguments before touching
+    the GPU; creation without a GPU returns NULL instead of exiting."""
+    import ctypes as C
+    import numpy as np
+    from amuse_amd.blocks import _bind
+    lib = _bind(C.CDLL(g6mod.LIBPATH))
+    k, u = C.c_double(), C.c_double()
+    assert lib.g6d_energy(None, C.byref(k), C.byref(u)) == -3
+    assert lib.g6d_run(None, 1.0, 0, 8, None) == -3
+    assert lib.g6d_set_precision(None, 0) == -3
+    assert lib.g6d_last_error(None) == b"null driver handle"
+    lib.g6d_free(None)
+    if not os.path.exists("/dev/kfd"):
+        m = np.ones(4) / 4
+        x = np.eye(4, 3)
+        D = C.POINTER(C.c_double)
+        h = lib.g6d_create(4, None, m.ctypes.data_as(D), x.ctypes.data_as(D),
+                           x.ctypes.data_as(D), 1e-4, 0.01, 0.0625, 0.03125, 8.0, 0, 1, 0)
+        assert h is None

@@ -129,6 +129,29 @@ def test_plummer_4096_block_sizes(g6, compat):
         assert_plain_parity(acc, jerk, pot, exact)
 
 
+def test_mixed_nn_ties_on_lattice(g6):
+    """On the integer lattice every r^2 is exact in FP32, so distance ties
+    are exact in the mixed path too: its nearest neighbour must be the
+    reference's (first slot wins) in every tile shape, per-pair compare
+    (scalar-load kernels) and running minimum with tile rescan (LDS) alike."""
+    d = load_golden("g6ref_lattice")
+    g6.reset()
+    upload(g6, d["x"], d["v"], d["a2"], d["j6"], d["k18"], d["m"], d["tj"], d["id_j"])
+    g6.set_ti(float(d["ti"]))
+    g6.set_precision(1)
+    nj = int(d["nj"])
+    try:
+        for njs, ipl, nchunk, smem in [(1, 2, 0, 1), (1, 1, 0, 0), (1, 2, 0, 0), (1, 4, 0, 0),
+                                       (4, 1, 0, 0), (16, 1, 0, 0), (64, 1, 0, 0),
+                                       (1, 2, 1, 0), (8, 2, 1, 0)]:
+            g6.set_tiles(njs, ipl, nchunk, smem)
+            nn = g6.calc(nj, d["id_i"], d["xi"], d["vi"], float(d["eps2"]))[3]
+            assert np.array_equal(nn, d["nn"]), (njs, ipl, nchunk, smem)
+    finally:
+        g6.set_tiles(0, 0)
+        g6.set_precision(0)
+
+
 def test_tile_shapes_agree(g6):
     s = plummer_state(3000, 22)
     upload(g6, s["x"], s["v"], s["a2"], s["j6"], s["k18"], s["m"], s["tj"], s["ids"])
@@ -140,7 +163,7 @@ def test_tile_shapes_agree(g6):
     _, _, _, onn, _ = oracle.force(id_i, xi, vi, 1e-4, s["ids"], s["m"], xp, vp)
     shapes = [(1, 2, 1), (1, 1, 1), (1, 2, 0), (1, 1, 0), (2, 2, 0), (2, 1, 0), (4, 2, 0),
               (4, 1, 0), (8, 2, 0), (8, 1, 0), (16, 1, 0), (32, 1, 0), (64, 1, 0), (128, 1, 0),
-              (256, 1, 0)]
+              (256, 1, 0), (1, 4, 0), (2, 4, 0)]
     for prec in (0, 1):
         g6.set_precision(prec)
         first = None
@@ -153,11 +176,16 @@ def test_tile_shapes_agree(g6):
                     assert np.array_equal(nn, onn), (njs, ipl, nchunk, smem)
                     assert_plain_parity(acc, jerk, pot, exact)
                 else:
-                    # mixed: every shape within FP32 rounding of the first
+                    # mixed: every shape within FP32 rounding of the first; the
+                    # FP32 r^2 of a pair does not depend on the shape, so the
+                    # nearest neighbour (per-pair compare in the scalar-load
+                    # kernels, running minimum + tile rescan in the LDS ones)
+                    # is the same slot in every shape
                     if first is None:
-                        first = (acc, pot)
+                        first = (acc, pot, nn)
                     scale = np.abs(first[0]).max()
                     assert np.abs(acc - first[0]).max() < 1e-5 * scale
+                    assert np.array_equal(nn, first[2]), (njs, ipl, nchunk, smem)
     g6.set_tiles(0, 0)
 
 
