@@ -118,43 +118,39 @@ __host__ __device__ inline int step_njs(int shape)
     return shape == 0 ? 1 : shape == 1 ? 4 : shape == 2 ? 16 : 64;
 }
 
-// Launch plan of one step from a small cost model in units of one pair on
-// one lane: the busiest lane of an item works ipl * ceil(jchunk / njs)
-// pairs, items run in rounds of 1024 resident workgroups (4 per CU), each
-// item costs a setup of ~16 pairs (tile loads, final shuffles), and a
-// j-split over workgroups (nchunk > 1) adds a reduce pass of ~512.
-// Splitting j across the lanes of a workgroup is free, so small blocks take
-// the 64-lanes-per-i shape without a reduce, large ones the (1, 2) shape.
+// Launch plan of one step.  Small systems (nj <= 2048, at most 8 tiles):
+// 64 lanes per i and no j-split over workgroups, so a step is one short
+// force kernel and no reduce.  Otherwise the rule of the ABI path's
+// automatic choice: the widest i-block not wider than the next power of two
+// of ni, then j-chunks for ~8 workgroups per CU within the partials budget
+// (measured better than 64 lanes per i from nj = 16384 on: with 4 i per
+// workgroup the j-tiles are loaded once per 4 i and the loads dominate).
 __host__ __device__ inline StepPlan plan_step(int ni, int nj)
 {
-    StepPlan best{3, 0, 1, kTJ};
-    long long best_cost = -1;
-    const int max_chunks = nj > kTJ ? (nj + kTJ - 1) / kTJ : 1;
-    for (int shape = 0; shape < 4; shape++) {
-        const int ib = step_ib(shape), njs = step_njs(shape);
-        const int ipl = shape == 0 ? 2 : 1;
-        const int iblocks = (ni + ib - 1) / ib;
-        for (int nchunk = 1; nchunk <= max_chunks; nchunk *= 2) {
-            if (nchunk > 1 && (long long)nchunk * ni > kStepPartBudget) break;
-            int jchunk = (nj + nchunk - 1) / nchunk;
-            jchunk = ((jchunk + kTJ - 1) / kTJ) * kTJ;
-            if (jchunk <= 0) jchunk = kTJ;
-            const int nch = nj > 0 ? (nj + jchunk - 1) / jchunk : 1;
-            const long long items = (long long)iblocks * nch;
-            const long long rounds = (items + 1023) / 1024;
-            const long long lane_pairs = (long long)ipl * ((jchunk + njs - 1) / njs);
-            const long long cost = rounds * (lane_pairs + 16) + (nch > 1 ? 512 : 0);
-            if (best_cost < 0 || cost < best_cost) {
-                best_cost = cost;
-                best.shape = shape;
-                best.iblocks = iblocks;
-                best.nchunk = nch;
-                best.jchunk = jchunk;
-            }
-            if (nch == 1 && nchunk > 1) break;
-        }
+    StepPlan sp;
+    if (nj <= 8 * kTJ) {
+        sp.shape = 3;
+        sp.iblocks = (ni + step_ib(3) - 1) / step_ib(3);
+        sp.nchunk = 1;
+        sp.jchunk = nj > 0 ? ((nj + kTJ - 1) / kTJ) * kTJ : kTJ;
+        return sp;
     }
-    return best;
+    int p2 = 1;
+    while (p2 < ni) p2 *= 2;
+    sp.shape = 3;
+    for (int k = 0; k < 4; k++)
+        if (step_ib(k) <= p2) { sp.shape = k; break; }
+    const int ib = step_ib(sp.shape);
+    sp.iblocks = (ni + ib - 1) / ib;
+    const int max_chunks = (nj + kTJ - 1) / kTJ;
+    int nchunk = sp.iblocks > 0 ? (2048 + sp.iblocks - 1) / sp.iblocks : 1;
+    if (nchunk > max_chunks) nchunk = max_chunks;
+    while (nchunk > 1 && (long long)nchunk * ni > kStepPartBudget) nchunk /= 2;
+    int jchunk = (nj + nchunk - 1) / nchunk;
+    jchunk = ((jchunk + kTJ - 1) / kTJ) * kTJ;
+    sp.jchunk = jchunk;
+    sp.nchunk = (nj + jchunk - 1) / jchunk;
+    return sp;
 }
 
 // Predicts every j-slot to the step's block time c->tnext (at_system = 0)

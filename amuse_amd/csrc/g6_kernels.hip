@@ -483,9 +483,10 @@ struct IF {
 struct AccF {
     // sub-tile FP32 sums
     float ax, ay, az, jx, jy, jz, pot;
-    float best;   // nearest-neighbour r^2 so far (tiles before the current one)
-    int addr;     // its slot
+    float best;   // nearest-neighbour r^2 so far (blocks before the current one)
+    int addr;     // its slot (found at the end, from blo/bhi)
     float tbest;  // running minimum of r^2 over the current block of j
+    int blo, bhi; // slot range of the lane's block that holds best
 };
 // double-single accumulator (hi + lo), fed with TwoSum
 struct DSAcc {
@@ -516,9 +517,9 @@ __device__ __forceinline__ float pair_r2_mx(const IF &I, float4 ja, float4 jb, f
 }
 
 // NNM: 0 no nearest neighbour; 1 per-pair compare, first slot wins; 2 only a
-// running minimum of r^2 over a block of j, the caller finding its slot by
-// a rescan of the block when the minimum improved (rare after the first
-// blocks), which takes the compare and two selects out of the pair.
+// running minimum of r^2 over a block of j: the caller keeps the block that
+// first reached the overall minimum and finds the slot in it once, at the
+// end, which takes the compare and two selects out of the pair.
 template <int NNM>
 __device__ __forceinline__ void pair_mx(const IF &I, AccF &a, float4 ja, float4 jb, float4 jc,
                                         int addr, float eps2)
@@ -593,7 +594,7 @@ __device__ __forceinline__ void force_mx_item(const FParams &p, TileF &tile, int
         I[q].hz = float(z); I[q].lz = float(z - double(I[q].hz));
         I[q].vx = float(p.ivx[i]); I[q].vy = float(p.ivy[i]); I[q].vz = float(p.ivz[i]);
         I[q].id = p.iid[i];
-        A[q] = AccF{0, 0, 0, 0, 0, 0, 0, __builtin_inff(), -1, __builtin_inff()};
+        A[q] = AccF{0, 0, 0, 0, 0, 0, 0, __builtin_inff(), -1, __builtin_inff(), 0, 0};
 #pragma unroll
         for (int k = 0; k < 7; k++) { D[q].h[k] = 0.0f; D[q].l[k] = 0.0f; }
     }
@@ -623,26 +624,34 @@ __device__ __forceinline__ void force_mx_item(const FParams &p, TileF &tile, int
 #pragma unroll
             for (int q = 0; q < IPL; q++) flush_mx(A[q], D[q]);
             if (NN) {
-                // A minimum over this block of the lane's j below the best
-                // so far (strictly: an equal r^2 keeps its earlier slot):
-                // its slot is the first j of the block with that r^2.  The
-                // block is short (kFlush pairs), so the rescan stays cheap
-                // however often the minimum improves.
+                // A block minimum below the best so far (strictly: an equal
+                // r^2 keeps the earlier block) moves the best to this block.
 #pragma unroll
                 for (int q = 0; q < IPL; q++) {
-                    const float tbest = A[q].tbest;
-                    if (tbest < A[q].best) {
-                        for (int jj = jb; jj < je; jj += NJS) {
-                            float dx, dy, dz;
-                            if (pair_r2_mx(I[q], tile.a[jj], tile.b[jj], tile.c[jj], dx, dy,
-                                           dz) == tbest) {
-                                A[q].addr = tb + jj;
-                                break;
-                            }
-                        }
-                        A[q].best = tbest;
-                    }
+                    const bool imp = A[q].tbest < A[q].best;
+                    A[q].best = imp ? A[q].tbest : A[q].best;
+                    A[q].blo = imp ? tb + jb : A[q].blo;
+                    A[q].bhi = imp ? tb + je : A[q].bhi;
                     A[q].tbest = __builtin_inff();
+                }
+            }
+        }
+    }
+    if (NN) {
+        // The slot: the first j of the best block, in the lane's stride,
+        // with exactly that r^2 (recomputed from the same arrays the tiles
+        // were loaded from, so bit-identical).
+#pragma unroll
+        for (int q = 0; q < IPL; q++) {
+            if (!(A[q].best < __builtin_inff())) continue;
+            for (int j = A[q].blo; j < A[q].bhi; j += NJS) {
+                const float4 a = make_float4(p.hx[j], p.hy[j], p.hz[j], 0.0f);
+                const float4 b = make_float4(p.lx[j], p.ly[j], p.lz[j], 0.0f);
+                const float4 c = make_float4(0.0f, 0.0f, __int_as_float(p.jid[j]), 0.0f);
+                float dx, dy, dz;
+                if (pair_r2_mx(I[q], a, b, c, dx, dy, dz) == A[q].best) {
+                    A[q].addr = j;
+                    break;
                 }
             }
         }
@@ -811,7 +820,7 @@ __global__ void __launch_bounds__(kWG) k_force_mx_s(FParams p)
         I[q].hz = float(z); I[q].lz = float(z - double(I[q].hz));
         I[q].vx = float(p.ivx[i]); I[q].vy = float(p.ivy[i]); I[q].vz = float(p.ivz[i]);
         I[q].id = p.iid[i];
-        A[q] = AccF{0, 0, 0, 0, 0, 0, 0, __builtin_inff(), -1, __builtin_inff()};
+        A[q] = AccF{0, 0, 0, 0, 0, 0, 0, __builtin_inff(), -1, __builtin_inff(), 0, 0};
 #pragma unroll
         for (int k = 0; k < 7; k++) { D[q].h[k] = 0.0f; D[q].l[k] = 0.0f; }
     }
