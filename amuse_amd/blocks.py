@@ -79,7 +79,10 @@ class DeviceHermite4:
         return rc
 
     def evolve(self, t_end: float, max_blocks: int = 0) -> bool:
-        """Block steps up to t_end; False if max_blocks (> 0) ran out first."""
+        """Block steps while the system time is below t_end, as ph4's
+        evolve_model (src/amuse_ph4/interface.cc:614-674; the time ends past
+        t_end when t_end is not commensurate with the steps); False if
+        max_blocks (> 0) ran out first."""
         reached = C.c_int(0)
         self._check(self.lib.g6d_run(self.h, float(t_end), int(max_blocks),
                                      self.steps_per_graph, C.byref(reached)), "g6d_run")

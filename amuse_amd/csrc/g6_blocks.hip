@@ -25,8 +25,8 @@
 // described with g6k::StepCtl); g6d_run captures
 // `steps_per_graph` of them into a hipGraph and replays it, reading the
 // control block (done flag, counters) back after each replay.  Kernels of a
-// finished run return at once, so a replay never overshoots t_end or the
-// block budget.
+// finished run return at once, so a replay takes no step after the end
+// condition (system time >= t_end, or the block budget spent).
 #include "../../include/g6x.h"
 #include "g6_kernels.h"
 #include "g6_predict.h"
@@ -91,6 +91,13 @@ __global__ void __launch_bounds__(256) kb_tmin(const double *__restrict__ t,
 {
     if (c->done) return;
     const bool first = blockIdx.x == 0 && threadIdx.x == 0;
+    // ph4's evolve_model loop runs while system_time < t_end
+    // (src/amuse_ph4/interface.cc:673-674); every thread reads the same
+    // system_time and t_end, which no kernel of this launch writes.
+    if (!(c->system_time < c->t_end)) {
+        if (first) c->done = 1;  // t_end reached
+        return;
+    }
     if (c->budget == 0) {
         if (first) c->done = 2;  // block budget spent
         return;
@@ -114,10 +121,6 @@ __global__ void __launch_bounds__(256) kb_select(State s, JStore js, IBlock b, S
 {
     if (c->done) return;
     const double tnext = __longlong_as_double(static_cast<long long>(c->tmin));
-    if (tnext > c->t_end) {
-        if (blockIdx.x == 0 && threadIdx.x == 0) c->done = 1;  // t_end reached
-        return;
-    }
     if (blockIdx.x == 0 && threadIdx.x == 0) c->tnext = tnext;
     for (int j = blockIdx.x * blockDim.x + threadIdx.x; j < s.n; j += gridDim.x * blockDim.x) {
         {

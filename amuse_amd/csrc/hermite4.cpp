@@ -297,16 +297,17 @@ H4_API int h4_set_ph4_initial_load(void *h, int on)
     return 0;
 }
 
-// Block steps until the next block time would pass t_end (t_end itself is
-// reached when it is commensurate with the steps, e.g. any dyadic time).
-// Returns 0 then, or 1 when max_blocks (> 0) block steps ran first.
+// Block steps while the system time is below t_end, as ph4's evolve_model
+// loop (src/amuse_ph4/interface.cc:614-674): on return the system time is
+// >= t_end, the last block ending on t_end when t_end is commensurate with
+// the steps (any dyadic time) and past it otherwise; particles are not
+// synchronised.  Returns 0 then, or 1 when max_blocks (> 0) block steps ran
+// first.
 H4_API int h4_evolve(void *h, double t_end, long long max_blocks)
 {
     Sim &s = *static_cast<Sim *>(h);
     for (long long done = 0;; done++) {
-        double tnext = 1e300;
-        for (int j = 0; j < s.n; j++) tnext = std::min(tnext, s.time[j] + s.step[j]);
-        if (tnext > t_end) return 0;
+        if (!(s.system_time < t_end)) return 0;
         if (max_blocks > 0 && done >= max_blocks) return 1;
         int rc = block_step(s);
         if (rc < 0) return rc;

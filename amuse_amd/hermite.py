@@ -77,7 +77,10 @@ class Hermite4:
         self.m = m
 
     def evolve(self, t_end: float, max_blocks: int = 0) -> bool:
-        """Block steps up to t_end; False if max_blocks (> 0) ran out first."""
+        """Block steps while the system time is below t_end, as ph4's
+        evolve_model (src/amuse_ph4/interface.cc:614-674; the time ends past
+        t_end when t_end is not commensurate with the steps); False if
+        max_blocks (> 0) ran out first."""
         rc = self.L.h4_evolve(self.h, float(t_end), int(max_blocks))
         if rc < 0:
             raise RuntimeError(f"h4_evolve failed ({rc})")

@@ -150,7 +150,9 @@ int g6a_adjoint_f64(int n, const double *x, const double *v, const double *m,
  * 0.0625, dt_limit 0.03125, dt_limitm 8) and returns a handle (NULL on
  * failure).  ph4_initial_load = 1 keeps acc = jerk = 0 in the j-set until
  * a particle's first step, as ph4 does.
- * g6d_run advances block steps while the next block time is <= t_end, or
+ * g6d_run advances block steps while the system time is below t_end (ph4's
+ * evolve_model loop, src/amuse_ph4/interface.cc:673-674: the system time
+ * ends >= t_end, past it when t_end is not commensurate with the steps), or
  * for max_blocks (> 0) block steps, replaying `steps_per_graph` captured
  * steps at a time; returns the block steps taken or a negative code, and
  * sets *reached (if not NULL) to 1 when it stopped at t_end, 0 when the

@@ -8,8 +8,9 @@ measured or shipped.  The product library never imports this package.
     libcpubest.so      ph4's CPU pair loop (idata.cc:195-236) vectorised and
                        threaded: the best-effort CPU row of the benchmark
     _ref/libg6ref.so   the reference lib/g6lib/g6lib.c compiled from its
-                       source by oracle/Makefile (this container; travels to
-                       the GPU box as a built file, never as source)
+                       source by oracle/Makefile where the reference checkout
+                       exists; elsewhere only the built file is used, never
+                       the source
 
 Arrays follow the ABI: [n][3] float64 vectors, int32 ids.
 """

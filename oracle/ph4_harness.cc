@@ -9,7 +9,8 @@
 //   library, loads every j: gpu.cc:36-92) -> new idata (initial forces) ->
 //   force_initial_timestep(0.0625, 0.03125, 8) -> new scheduler ->
 //   repeated jdata::advance until system_time >= t_out, energies
-//   from jdata::get_kin / get_pot (a full pass through the library).
+//   from jdata::get_kin / get_pot (a full pass through the library) and,
+//   when dtmin is not NULL, the smallest particle time step.
 //
 // Encounter management is off (manage_encounters = 0) so only the Hermite
 // block-step path runs.
@@ -20,7 +21,7 @@
 extern "C" __attribute__((visibility("default"))) int
 ph4h_run(int n, const double *m, const double *x, const double *v, double eps2, double eta,
          int nout, const double *t_out, double *kin, double *pot, long long *blocks,
-         long long *steps)
+         long long *steps, double *dtmin)
 {
     jdata *jd = new jdata;
     jd->setup_gpu();
@@ -43,6 +44,11 @@ ph4h_run(int n, const double *m, const double *x, const double *v, double eps2, 
         pot[k] = jd->get_pot();
         blocks[k] = jd->block_steps;
         steps[k] = jd->total_steps;
+        if (dtmin) {
+            double lo = jd->timestep[0];
+            for (int j = 1; j < jd->nj; j++) lo = jd->timestep[j] < lo ? jd->timestep[j] : lo;
+            dtmin[k] = lo;
+        }
     }
     const int used_library = jd->use_gpu ? 1 : 0;
     delete s;

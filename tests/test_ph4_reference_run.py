@@ -1,7 +1,7 @@
 """The reference ph4 integrator, built unchanged from src/amuse_ph4/src with
 -DGPU (`make -C oracle ph4run`), runs on a GRAPE-6 library of our choosing.
 
-CPU (this container): on the CPU oracle behind the ABI, our driver
+CPU: on the CPU oracle behind the ABI, our driver
 amuse_amd/csrc/hermite4.cpp reproduces ph4 bit for bit — the same block and
 particle step counts and the same energies to the last bit — which pins the
 driver to the reference integrator.
