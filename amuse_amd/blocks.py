@@ -27,6 +27,7 @@ def _bind(lib):
         "g6d_run": (C.c_longlong, [C.c_void_p, C.c_double, C.c_longlong, C.c_int, _I]),
         "g6d_energy": (C.c_int, [C.c_void_p, _D, _D]),
         "g6d_state": (C.c_int, [C.c_void_p, _D, _D]),
+        "g6d_state_raw": (C.c_int, [C.c_void_p, _D, _D]),
         "g6d_steps": (C.c_int, [C.c_void_p, _D, _D]),
         "g6d_counters": (C.c_int, [C.c_void_p, _LL, _LL, _D]),
         "g6d_set_precision": (C.c_int, [C.c_void_p, C.c_int]),
@@ -102,6 +103,15 @@ class DeviceHermite4:
         v = np.zeros((self.n, 3))
         self._check(self.lib.g6d_state(self.h, x.ctypes.data_as(_D), v.ctypes.data_as(_D)),
                     "g6d_state")
+        return x, v
+
+    def state_raw(self):
+        """x and v as last corrected, each at its particle's own time (ph4's
+        get_state, src/amuse_ph4/interface.cc:485-502)."""
+        x = np.zeros((self.n, 3))
+        v = np.zeros((self.n, 3))
+        self._check(self.lib.g6d_state_raw(self.h, x.ctypes.data_as(_D), v.ctypes.data_as(_D)),
+                    "g6d_state_raw")
         return x, v
 
     def timesteps(self):

@@ -1,6 +1,6 @@
 """Builds the native libraries of amuse_amd in-tree with hipcc (gfx950 only).
 
-    python -m amuse_amd.build          # libg6.so
+    python -m amuse_amd.build          # libg6.so, libhermite4.so, g6_worker
     python -m amuse_amd.build --isa    # also dump the gfx950 assembly to _build/
 
 The libraries land in amuse_amd/lib/ so they travel with the source tree; no
@@ -30,6 +30,10 @@ LIBS = {
 # host-only C++ (no device code): built with g++
 CPU_LIBS = {
     "libhermite4.so": (["hermite4.cpp"], ["-ldl"]),
+}
+# host-only executables linked against libg6.so (found next to them at run time)
+EXES = {
+    "g6_worker": ["g6_worker.cpp"],
 }
 
 
@@ -80,8 +84,24 @@ def build_cpu_lib(name: str) -> str:
     return out
 
 
+def rocm_libdir() -> str:
+    return os.path.join(os.path.dirname(os.path.dirname(os.path.realpath(hipcc()))), "lib")
+
+
+def build_exe(name: str) -> str:
+    os.makedirs(LIBDIR, exist_ok=True)
+    out = os.path.join(LIBDIR, name)
+    tmp = out + ".tmp"
+    _run(["g++", "-O2", "-std=c++17", "-Wall", "-I", os.path.join(ROOT, "include"),
+          *[os.path.join(CSRC, s) for s in EXES[name]], "-L", LIBDIR, "-lg6",
+          "-Wl,-rpath,$ORIGIN", "-Wl,-rpath-link," + rocm_libdir(), "-o", tmp])
+    os.replace(tmp, out)
+    return out
+
+
 def build_all(isa: bool = False) -> list[str]:
-    return [build_lib(n, isa) for n in LIBS] + [build_cpu_lib(n) for n in CPU_LIBS]
+    return ([build_lib(n, isa) for n in LIBS] + [build_cpu_lib(n) for n in CPU_LIBS]
+            + [build_exe(n) for n in EXES])
 
 
 if __name__ == "__main__":

@@ -554,6 +554,30 @@ G6D_API int g6d_energy(void *h, double *kin, double *pot)
     return 0;
 }
 
+// Positions and velocities as last corrected, each at its particle's own
+// time ([n][3] each; ph4's get_state, src/amuse_ph4/interface.cc:485-502).
+G6D_API int g6d_state_raw(void *h, double (*x)[3], double (*v)[3])
+{
+    if (!h || !x || !v) return G6X_ERR_ARG;
+    Driver *D = static_cast<Driver *>(h);
+    g6b::State &s = D->st;
+    const size_t nd = size_t(s.n);
+    if (g6b::read_ctl(D)) return G6X_ERR_HIP;
+    std::vector<double> col(6 * nd);
+    double *rows[6] = {s.x[0], s.x[1], s.x[2], s.v[0], s.v[1], s.v[2]};
+    for (int r = 0; r < 6; r++)
+        if (hipMemcpyAsync(col.data() + r * nd, rows[r], nd * 8, hipMemcpyDeviceToHost,
+                           D->stream) != hipSuccess)
+            return G6X_ERR_HIP;
+    if (hipStreamSynchronize(D->stream) != hipSuccess) return G6X_ERR_HIP;
+    for (size_t i = 0; i < nd; i++)
+        for (int k = 0; k < 3; k++) {
+            x[i][k] = col[k * nd + i];
+            v[i][k] = col[(3 + k) * nd + i];
+        }
+    return 0;
+}
+
 // Positions and velocities predicted to the system time ([n][3] each).
 G6D_API int g6d_state(void *h, double (*x)[3], double (*v)[3])
 {

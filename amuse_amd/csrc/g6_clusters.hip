@@ -61,6 +61,10 @@ struct Dev {
     double *energy = nullptr;  // [nc]
     int *item_c = nullptr, *item_i0 = nullptr, *item_njs = nullptr;
     int nitems = 0;
+    // particles of the clusters the work items cover (the predictor and
+    // corrector run over these only, so retired clusters cost nothing)
+    int *plist = nullptr;
+    int nlist = 0;
     int *queue = nullptr, *nactive = nullptr;
     double t_end = 1.0, dt_param = 0.03, eps2 = 1e-6, dt_limit = 0.03125, dt_floor = 0x1p-40;
 };
@@ -75,9 +79,10 @@ __device__ __forceinline__ unsigned long long dbits(double x)
 // ---------------------------------------------------------------------------
 __global__ void __launch_bounds__(256) kc_predict(Dev d, int zero_step)
 {
-    const int p = blockIdx.x * blockDim.x + threadIdx.x;
-    if (p == 0) *d.queue = 0;
-    if (p >= d.np) return;
+    const int q = blockIdx.x * blockDim.x + threadIdx.x;
+    if (q == 0) *d.queue = 0;
+    if (q >= d.nlist) return;
+    const int p = d.plist[q];
     const int c = d.pc[p];
     if (!d.active[c]) return;
     const double dt = zero_step ? 0.0 : d.dt[c];
@@ -289,8 +294,9 @@ __global__ void __launch_bounds__(kWG) kc_force(Dev d)
 // ---------------------------------------------------------------------------
 __global__ void __launch_bounds__(256) kc_correct(Dev d)
 {
-    const int p = blockIdx.x * blockDim.x + threadIdx.x;
-    if (p >= d.np) return;
+    const int q = blockIdx.x * blockDim.x + threadIdx.x;
+    if (q >= d.nlist) return;
+    const int p = d.plist[q];
     const int c = d.pc[p];
     if (!d.active[c]) return;
     const double dt = d.dt[c];
@@ -430,18 +436,41 @@ int size_class(int n)
 // ones.  `only` (host flags, one per cluster) restricts the list to some
 // clusters: every workgroup pulls items one atomic at a time, so items of
 // retired clusters would cost a serial chain of atomics per step.
-int build_items(Ensemble *E, const int *only = nullptr)
+// fill: when the covered clusters give fewer than two items per compute unit
+// (the last clusters of a run, often with the smallest steps), raise the
+// lanes per i (up to 64) until they do, so a step's latency is not that of
+// a few long items.
+int build_items(Ensemble *E, const int *only = nullptr, bool fill = false)
 {
     // the previous item arrays may still be read by queued work
     CK(hipStreamSynchronize(E->stream));
-    std::vector<int> ic, ii0, injs;
+    auto njs_of = [&](int n, int boost) {
+        return std::min(64, E->njs_class[size_class(n)] * boost);
+    };
+    int boost = 1;
+    if (fill) {
+        const int target = std::max(1, E->persistent_wgs / 2);
+        for (; boost < 64; boost *= 2) {
+            long long count = 0;
+            bool room = false;
+            for (int c = 0; c < E->d.nc; c++) {
+                if (only && !only[c]) continue;
+                const int rows = kWG / njs_of(E->cnt[c], boost);
+                count += (E->cnt[c] + rows - 1) / rows;
+                room |= njs_of(E->cnt[c], boost) < 64;
+            }
+            if (count >= target || !room) break;
+        }
+    }
+    std::vector<int> ic, ii0, injs, plist;
     std::vector<double> cost;
     int covered = 0;
     for (int c = 0; c < E->d.nc; c++) {
         if (only && !only[c]) continue;
         covered++;
         const int n = E->cnt[c];
-        const int njs = E->njs_class[size_class(n)];
+        for (int i = 0; i < n; i++) plist.push_back(E->off[c] + i);
+        const int njs = njs_of(n, boost);
         const int rows = kWG / njs;
         for (int i0 = 0; i0 < n; i0 += rows) {
             ic.push_back(c);
@@ -470,7 +499,10 @@ int build_items(Ensemble *E, const int *only = nullptr)
     CK(hipMemcpy(E->d.item_c, a.data(), sizeof(int) * n, hipMemcpyHostToDevice));
     CK(hipMemcpy(E->d.item_i0, b.data(), sizeof(int) * n, hipMemcpyHostToDevice));
     CK(hipMemcpy(E->d.item_njs, cc.data(), sizeof(int) * n, hipMemcpyHostToDevice));
+    if (!plist.empty())
+        CK(hipMemcpy(E->d.plist, plist.data(), sizeof(int) * plist.size(), hipMemcpyHostToDevice));
     E->d.nitems = n;
+    E->d.nlist = int(plist.size());
     E->items_clusters = covered;
     if (E->graph) {
         hipGraphExecDestroy(E->graph);
@@ -481,16 +513,23 @@ int build_items(Ensemble *E, const int *only = nullptr)
 
 int launch_force(Ensemble *E, int prec)
 {
+    // no more workgroups than items: the surplus would only take the atomic
+    // that tells it the queue is empty
+    const int wgs = std::max(1, std::min(E->persistent_wgs, E->d.nitems));
     if (prec == G6X_MIXED)
-        hipLaunchKernelGGL(kc_force<true>, dim3(E->persistent_wgs), dim3(kWG), 0, E->stream, E->d);
+        hipLaunchKernelGGL(kc_force<true>, dim3(wgs), dim3(kWG), 0, E->stream, E->d);
     else
-        hipLaunchKernelGGL(kc_force<false>, dim3(E->persistent_wgs), dim3(kWG), 0, E->stream, E->d);
+        hipLaunchKernelGGL(kc_force<false>, dim3(wgs), dim3(kWG), 0, E->stream, E->d);
     return hipGetLastError() == hipSuccess ? 0 : G6X_ERR_HIP;
 }
 
+// blocks of the predictor / corrector over the listed particles (one at least,
+// which resets the work queue)
+int list_blocks(const Ensemble *E) { return std::max(1, (E->d.nlist + 255) / 256); }
+
 int enqueue_step(Ensemble *E, int prec)
 {
-    const int pb = (E->d.np + 255) / 256, cb = (E->d.nc + 255) / 256;
+    const int pb = list_blocks(E), cb = (E->d.nc + 255) / 256;
     hipLaunchKernelGGL(kc_predict, dim3(pb), dim3(256), 0, E->stream, E->d, 0);
     if (launch_force(E, prec)) return G6X_ERR_HIP;
     hipLaunchKernelGGL(kc_correct, dim3(pb), dim3(256), 0, E->stream, E->d);
@@ -531,10 +570,10 @@ G6C_API void *g6c_create(int nc, const int counts[], const double m[], const dou
     for (int k = 0; k < kClasses; k++) E->njs_class[k] = k <= 1 ? 16 : (k <= 3 ? 4 : 1);
     if (hipStreamCreateWithFlags(&E->stream, hipStreamNonBlocking) != hipSuccess) { delete E; return nullptr; }
     // one allocation: per-particle doubles (x v a j px pv na nj = 24, m, npot), floats (10),
-    // ints (pc), per-cluster doubles (t dt energy) + uint (cmax) + ints (off cnt active nsteps)
+    // ints (pc plist), per-cluster doubles (t dt energy) + uint (cmax) + ints (off cnt active nsteps)
     const size_t P = size_t(np), Cn = size_t(nc);
     auto al = [](size_t b) { return (b + 255) & ~size_t(255); };
-    size_t bytes = al(P * 8) * 26 + al(P * 4) * 11 + al(Cn * 8) * 4 + al(Cn * 4) * 4 + 256;
+    size_t bytes = al(P * 8) * 26 + al(P * 4) * 12 + al(Cn * 8) * 4 + al(Cn * 4) * 4 + 256;
     if (hipMalloc(&E->base, bytes) != hipSuccess) { delete E; return nullptr; }
     char *q = static_cast<char *>(E->base);
     auto D = [&](double *&p) { p = reinterpret_cast<double *>(q); q += al(P * 8); };
@@ -544,6 +583,7 @@ G6C_API void *g6c_create(int nc, const int counts[], const double m[], const dou
     for (int k = 0; k < 3; k++) { F(d.hx[k]); F(d.lx[k]); F(d.fv[k]); }
     F(d.fm);
     d.pc = reinterpret_cast<int *>(q); q += al(P * 4);
+    d.plist = reinterpret_cast<int *>(q); q += al(P * 4);
     d.t = reinterpret_cast<double *>(q); q += al(Cn * 8);
     d.dt = reinterpret_cast<double *>(q); q += al(Cn * 8);
     d.energy = reinterpret_cast<double *>(q); q += al(Cn * 8);
@@ -603,8 +643,7 @@ static int force_now(Ensemble *E, int precision)
 {
     // a run may have dropped the items of retired clusters
     if (E->items_clusters < E->d.nc && build_items(E)) return G6X_ERR_HIP;
-    const int pb = (E->d.np + 255) / 256;
-    hipLaunchKernelGGL(kc_predict, dim3(pb), dim3(256), 0, E->stream, E->d, 1);
+    hipLaunchKernelGGL(kc_predict, dim3(list_blocks(E)), dim3(256), 0, E->stream, E->d, 1);
     if (launch_force(E, precision)) return G6X_ERR_HIP;
     return 0;
 }
@@ -686,7 +725,7 @@ G6C_API long long g6c_run(void *h, int steps_per_graph, int use_graph, long long
         if (*h_active * 2 <= E->items_clusters) {
             // half the covered clusters have retired: drop their items
             e = hipMemcpy(flags.data(), E->d.active, sizeof(int) * E->d.nc, hipMemcpyDeviceToHost);
-            if (e != hipSuccess || build_items(E, flags.data())) {
+            if (e != hipSuccess || build_items(E, flags.data(), true)) {
                 if (e != hipSuccess) E->err = std::string("active flags: ") + hipGetErrorString(e);
                 rc = G6X_ERR_HIP;
                 break;
@@ -807,11 +846,11 @@ G6C_API int g6c_autotune(void *h)
         for (int njs = 1; njs <= 64; njs *= 2) {
             E->njs_class[k] = njs;
             if (build_items(E, act.data())) return G6X_ERR_HIP;
-            hipLaunchKernelGGL(kc_predict, dim3((E->d.np + 255) / 256), dim3(256), 0, E->stream, E->d, 1);
+            hipLaunchKernelGGL(kc_predict, dim3(list_blocks(E)), dim3(256), 0, E->stream, E->d, 1);
             launch_force(E, E->precision);
             hipEventRecord(a, E->stream);
             for (int r = 0; r < 3; r++) {
-                hipLaunchKernelGGL(kc_predict, dim3((E->d.np + 255) / 256), dim3(256), 0, E->stream, E->d, 1);
+                hipLaunchKernelGGL(kc_predict, dim3(list_blocks(E)), dim3(256), 0, E->stream, E->d, 1);
                 launch_force(E, E->precision);
             }
             hipEventRecord(b, E->stream);

@@ -329,6 +329,68 @@ def driver_study(args) -> int:
     return 0
 
 
+def worker_study(args) -> int:
+    """The device-resident integrator out of process through AMUSE's socket
+    protocol (amuse_amd/lib/g6_worker): round trip of a scalar call, get_state
+    of N = 65536 particles in one vectorised call, and ph4's c0 loop (Plummer
+    N = 1024, eps2 = 1e-4, eta = 0.01, t = 0..1/8 in 8 evolve_model calls)
+    through the worker against the same run in process."""
+    from amuse_amd import ic
+    from amuse_amd.blocks import DeviceHermite4
+    from amuse_amd.worker import Ph4Worker
+    res = {"workload": "socket worker (AMUSE message protocol) around the device driver"}
+    w = Ph4Worker()
+    try:
+        w.call("initialize_code")
+        t0 = time.perf_counter()
+        for _ in range(2000):
+            w.call("get_eps2")
+        res["scalar_call_us"] = (time.perf_counter() - t0) / 2000 * 1e6
+        log(f"  worker: scalar call {res['scalar_call_us']:.1f} us")
+        n = 65536
+        rng = np.random.Generator(np.random.PCG64(9))
+        m, x, v = np.full(n, 1.0 / n), rng.normal(size=(n, 3)), rng.normal(size=(n, 3))
+        r, out = w.call("new_particle", m, *x.T, *v.T, np.zeros(n), -1)
+        ids = out["index_of_the_particle"]
+        t0 = time.perf_counter()
+        for _ in range(10):
+            w.call("get_state", ids)
+        dt = (time.perf_counter() - t0) / 10
+        nbytes = n * (4 + 4 + 8 * 8)  # index in, result + 8 doubles out
+        res["get_state_65536"] = {"ms": dt * 1e3, "MB_per_s": nbytes / dt / 1e6}
+        log(f"  worker: get_state of {n}: {res['get_state_65536']}")
+        w.call("cleanup_code")
+        m, x, v = ic.plummer(1024, 1024)
+        w.call("set_eps2", 1e-4)
+        w.call("set_eta", 0.01)
+        w.call("new_particle", m, *x.T, *v.T, np.zeros(1024), -1)
+        t0 = time.perf_counter()
+        if w.call("commit_particles")[0] != 0:
+            raise RuntimeError("worker: commit_particles failed (no GPU?)")
+        for k in range(1, 9):
+            if w.call("evolve_model", k / 64)[0] != 0:
+                raise RuntimeError("worker: evolve_model failed")
+        e_w = w.call("get_kinetic_energy")[1]["kinetic_energy"] + \
+            w.call("get_potential_energy")[1]["potential_energy"]
+        t_w = time.perf_counter() - t0
+        blocks = w.call("get_block_steps")[1]["block_steps"]
+    finally:
+        w.stop()
+    d = DeviceHermite4(precision=0)
+    t0 = time.perf_counter()
+    d.commit(m, x, v, eps2=1e-4, eta=0.01)
+    for k in range(1, 9):
+        d.evolve(k / 64)
+    e_d = sum(d.energy())
+    t_d = time.perf_counter() - t0
+    d.close()
+    res["c0_t_0_to_1_8"] = {"block_steps": blocks, "worker_s": t_w, "in_process_s": t_d,
+                            "energy_equal": bool(e_w == e_d)}
+    log(f"  worker: c0 to t=1/8: {res['c0_t_0_to_1_8']}")
+    print(json.dumps(res))
+    return 0
+
+
 def clusters_c2(args) -> int:
     """Configuration c2: 4096 clusters, N log-uniform in [16, 2048], Kroupa
     0.08-20 normalised to M = 1 per cluster, eps2 = 1e-6, all to t = 1 in
@@ -476,7 +538,8 @@ def main() -> int:
     ap.add_argument("--copy-calib", action="store_true",
                     help="only run a plain 1 GiB device copy (HBM counter calibration)")
     ap.add_argument("--workload", default="full",
-                    choices=["full", "blocks", "tiles", "ph4", "clusters", "adjoint", "driver"],
+                    choices=["full", "blocks", "tiles", "ph4", "clusters", "adjoint", "driver",
+                             "worker"],
                     help="full: full-N force pass at N=2^20 (default); blocks: i-blocks of "
                          "2^k against N=65536 with per-j times, FP64 vs mixed errors")
     args = ap.parse_args()
@@ -494,6 +557,8 @@ def main() -> int:
         return adjoint_c3(args)
     if args.workload == "driver":
         return driver_study(args)
+    if args.workload == "worker":
+        return worker_study(args)
     if args.gpus != 1:
         print("bench.py measures one GPU: the force path is built and measured on a single "
               "MI355X (multi-GPU j-domain sharding is a design note only)", file=sys.stderr)

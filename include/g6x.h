@@ -158,7 +158,9 @@ int g6a_adjoint_f64(int n, const double *x, const double *v, const double *m,
  * sets *reached (if not NULL) to 1 when it stopped at t_end, 0 when the
  * block budget ran out first.
  * g6d_energy: kinetic and potential energy at the system time (FP64 pass).
- * g6d_state: x and v predicted to the system time.  g6d_steps: per-particle
+ * g6d_state: x and v predicted to the system time.  g6d_state_raw: x and v
+ * as last corrected, at each particle's own time (ph4's get_state,
+ * src/amuse_ph4/interface.cc:485-502).  g6d_steps: per-particle
  * time and step.  g6d_counters: block steps, particle steps, system time. */
 void *g6d_create(int n, const int ids[], const double m[], const double x[][3],
                  const double v[][3], double eps2, double eta, double dt_fac,
@@ -168,6 +170,7 @@ long long g6d_run(void *drv, double t_end, long long max_blocks, int steps_per_g
                   int *reached);
 int g6d_energy(void *drv, double *kinetic, double *potential);
 int g6d_state(void *drv, double x[][3], double v[][3]);
+int g6d_state_raw(void *drv, double x[][3], double v[][3]);
 int g6d_steps(void *drv, double time[], double step[]);
 int g6d_counters(void *drv, long long *block_steps, long long *particle_steps,
                  double *system_time);

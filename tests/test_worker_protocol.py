@@ -1,0 +1,152 @@
+"""AMUSE's socket message protocol between amuse_amd/worker.py (the proxy)
+and amuse_amd/lib/g6_worker (the worker), without a GPU: function ids,
+message layout, vectorised calls, ph4's parameter and particle bookkeeping,
+error replies and the stop call.
+
+Pins: the crc32 known answer that AMUSE checks before trusting its id
+function (src/amuse/rfi/core.py:399, crc32("amuse") = 0xC0CC9367); the
+header layout of src/amuse/rfi/channel.py:1750-1790 and :1905-1921 (4 flag
+bytes, then call id, function id, call count and the seven array lengths as
+int32) and the string layout of :1843-1860 / :1923-1940 (int32 lengths, then
+the UTF-8 characters, each string followed by a NUL); ph4's id assignment
+(jdata.cc:190-230: 1, 2, ... unless a free id is asked for).  The worker
+computes its ids with its own CRC-32; every call below succeeding shows the
+two sides agree.
+"""
+import struct
+import zlib
+
+import numpy as np
+import pytest
+
+from amuse_amd.worker import SPECS, Message, Ph4Worker, WorkerError, function_id
+
+
+def test_function_id_known_answer():
+    assert zlib.crc32(b"amuse") & 0xFFFFFFFF == 0xC0CC9367
+    # signed reading of the crc, then the absolute value
+    for name in SPECS:
+        x = zlib.crc32(name.encode())
+        s = struct.unpack("<i", struct.pack("<I", x))[0]
+        assert function_id(name) == abs(s)
+    ids = [function_id(n) for n in SPECS]
+    assert len(set(ids)) == len(ids) and 0 not in ids
+
+
+def test_message_layout_and_roundtrip():
+    m = Message(7, 123456, 3, ints=[1, -2, 3], longs=[2 ** 40], floats=[1.5],
+                doubles=[0.1, 0.2], booleans=[1, 0], strings=["ab", "", "éx"],
+                units=[9.0], error=True)
+    b = m.encode()
+    head = struct.unpack("<4b10i", b[:44])
+    assert head[:4] == (0, 1, 1, 0)                       # endianness, error, units, -
+    assert head[4:] == (7, 123456, 3, 3, 1, 1, 2, 2, 3, 1)
+    at = 44 + 12 + 8 + 4 + 16 + 2
+    lens = struct.unpack("<3i", b[at:at + 12])
+    assert lens == (2, 0, 3)                              # bytes, not characters
+    assert b[at + 12:at + 12 + 8] == b"ab\0\0" + "éx".encode() + b"\0"
+    pos = [0]
+
+    def recv(n):
+        out = b[pos[0]:pos[0] + n]
+        pos[0] += n
+        return out
+
+    r = Message.read(recv)
+    assert pos[0] == len(b)
+    assert (r.call_id, r.function_id, r.call_count, r.error) == (7, 123456, 3, True)
+    assert r.ints.tolist() == [1, -2, 3] and r.longs.tolist() == [2 ** 40]
+    assert r.doubles.tolist() == [0.1, 0.2] and r.strings == ["ab", "", "éx"]
+    assert r.units.tolist() == [9.0] and r.booleans.tolist() == [1, 0]
+
+
+@pytest.fixture
+def worker():
+    w = Ph4Worker()
+    yield w
+    assert w.stop() == 0
+
+
+def test_parameters(worker):
+    assert worker.call("initialize_code") == (0, {})
+    # ph4's defaults (interface.py:506-628, interface.cc:29-31)
+    assert worker.call("get_eta")[1]["eta"] == 0.14
+    assert worker.call("get_eps2")[1]["epsilon_squared"] == 0.0
+    assert worker.call("get_initial_timestep_fac")[1]["fac"] == 0.0625
+    assert worker.call("get_initial_timestep_limit")[1]["limit"] == 0.03125
+    assert worker.call("get_initial_timestep_median")[1]["median"] == 8.0
+    assert worker.call("get_gpu")[1]["use_gpu"] == 1
+    for name, val in (("eps2", 1e-4), ("eta", 0.01), ("begin_time", 2.5),
+                      ("initial_timestep_fac", 0.125), ("initial_timestep_limit", 0.0625),
+                      ("initial_timestep_median", 4.0)):
+        assert worker.call("set_" + name, val) == (0, {})
+        r, out = worker.call("get_" + name)
+        assert r == 0 and list(out.values()) == [val]
+    assert worker.call("set_gpu_id", 0) == (0, {})
+    assert worker.call("get_gpu_id")[1]["gpu_id"] == 0
+    assert worker.call("set_force_precision", 5)[0] == -1
+    assert worker.call("set_force_precision", 1)[0] == 0
+    assert worker.call("get_force_precision")[1]["precision"] == 1
+    assert worker.call("commit_parameters")[0] == 0
+    # before commit_particles nothing is integrated
+    assert worker.call("get_time") == (0, {"time": 2.5})
+    assert worker.call("evolve_model", 1.0)[0] == -1
+    assert worker.call("get_kinetic_energy")[0] == -1
+
+
+def test_particles_vectorised(worker):
+    rng = np.random.default_rng(3)
+    n = 300
+    m = rng.uniform(0.1, 1.0, n)
+    x = rng.normal(size=(n, 3))
+    v = rng.normal(size=(n, 3))
+    rad = rng.uniform(0, 0.1, n)
+    r, out = worker.call("new_particle", m, *x.T, *v.T, rad, -1)
+    assert (r == 0).all()
+    ids = out["index_of_the_particle"]
+    assert ids.tolist() == list(range(1, n + 1))
+    # an explicit free id is kept, a taken one is replaced by the next free
+    r, out = worker.call("new_particle", 1.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 1000)
+    assert (r, out["index_of_the_particle"]) == (0, 1000)
+    r, out = worker.call("new_particle", 1.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 5)
+    assert (r, out["index_of_the_particle"]) == (0, n + 1)
+    assert worker.call("get_number_of_particles")[1]["number_of_particles"] == n + 2
+    r, st = worker.call("get_state", ids)
+    assert (r == 0).all()
+    assert np.array_equal(st["mass"], m) and np.array_equal(st["radius"], rad)
+    assert np.array_equal(np.stack([st["x"], st["y"], st["z"]], 1), x)
+    assert np.array_equal(np.stack([st["vx"], st["vy"], st["vz"]], 1), v)
+    r, p = worker.call("get_position", ids[::7])
+    assert np.array_equal(p["y"], x[::7, 1])
+    r, vv = worker.call("get_velocity", ids[::5])
+    assert np.array_equal(vv["vz"], v[::5, 2])
+    assert worker.call("get_total_mass")[1]["mass"] == pytest.approx(m.sum() + 2.0, rel=1e-15)
+    # unknown ids give -1 per call, the others still answer
+    r, out = worker.call("get_mass", np.array([1, 99999, 3], dtype=np.int32))
+    assert r.tolist() == [0, -1, 0] and out["mass"][[0, 2]].tolist() == [m[0], m[2]]
+    # deletion keeps the other ids
+    assert worker.call("delete_particle", 2)[0] == 0
+    assert worker.call("delete_particle", 2)[0] == -1
+    r, out = worker.call("get_mass", np.array([1, 3], dtype=np.int32))
+    assert out["mass"].tolist() == [m[0], m[2]]
+    assert worker.call("get_number_of_particles")[1]["number_of_particles"] == n + 1
+
+
+def test_errors(worker):
+    rep = worker.send(Message(11, 4242, 1))
+    assert rep.error and rep.strings == ["unknown function id: 4242\n"]
+    assert rep.call_id == 11 and rep.function_id == 4242
+    # wrong number of inputs for the function
+    rep = worker.send(Message(12, function_id("set_eps2"), 1, doubles=[1.0, 2.0]))
+    assert rep.error and "set_eps2" in rep.strings[0]
+    # the worker is still there
+    assert worker.call("get_eps2")[0] == 0
+
+
+def test_error_reply_raises(worker, monkeypatch):
+    from amuse_amd import worker as W
+    monkeypatch.setitem(W.SPECS, "no_such_function", ([], [], [], []))
+    monkeypatch.setitem(W.SPECS_ORDER, "no_such_function", [])
+    with pytest.raises(WorkerError, match="unknown function id"):
+        worker.call("no_such_function")
+    assert worker.call("get_number_of_particles")[1]["number_of_particles"] == 0
