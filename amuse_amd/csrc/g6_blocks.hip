@@ -20,16 +20,20 @@
 // limit (jdata.cc:503-564) computed on the host from the downloaded
 // acc/jerk, the same arithmetic as the host driver (h4_rules.h).
 //
-// One step is five kernels (next-time reduction, selection with the i- and
-// j-prediction, force, reduce, correction; the control-block protocol is
-// described with g6k::StepCtl); g6d_run captures
-// `steps_per_graph` of them into a hipGraph and replays it, reading the
+// One step is three kernels: selection (end test, block time, the block
+// with its i-prediction, the j-prediction, and the next block time's
+// minimum over the particles not in the block), force, and correction (the
+// fold of the force partials, the corrector, the new t + dt into the next
+// minimum); the control-block protocol is described with g6k::StepCtl.
+// g6d_run captures `steps_per_graph` of them (rounded up to even: the step
+// parity is a launch argument) into a hipGraph and replays it, reading the
 // control block (done flag, counters) back after each replay.  Kernels of a
 // finished run return at once, so a replay takes no step after the end
 // condition (system time >= t_end, or the block budget spent).
 #include "../../include/g6x.h"
 #include "g6_kernels.h"
 #include "g6_predict.h"
+#include "g6_reduce.h"
 #include "h4_rules.h"
 
 #include <algorithm>
@@ -83,17 +87,54 @@ inline int nblk(int n, int b) { return (n + b - 1) / b; }
 // ---------------------------------------------------------------------------
 // step kernels
 // ---------------------------------------------------------------------------
-// Next block time: min over particles of t + dt (wave shuffles, then one
-// 64-bit atomicMin per wave on the order-preserving bits).  Also ends the
-// run when the block budget is spent and resets the block counter.
-__global__ void __launch_bounds__(256) kb_tmin(const double *__restrict__ t,
-                                               const double *__restrict__ dt, int n, StepCtl *c)
+constexpr unsigned long long kInfBits = 0x7ff0000000000000ull;  // +inf
+
+// Folds a lane's candidate next block time into *slot: wave shuffles, then
+// one 64-bit atomicMin per wave on the order-preserving bits of the
+// (positive) times.
+__device__ __forceinline__ void fold_min(double lo, unsigned long long *slot)
+{
+#pragma unroll
+    for (int off = 32; off >= 1; off >>= 1) lo = fmin(lo, __shfl_xor(lo, off));
+    if ((threadIdx.x & 63) == 0 && lo < __builtin_inf())
+        atomicMin(slot, static_cast<unsigned long long>(__double_as_longlong(lo)));
+}
+
+// Control block for a fresh run of steps: no block time yet, counts zero.
+__global__ void kb_ctl_reset(StepCtl *c)
+{
+    c->tmin[0] = c->tmin[1] = kInfBits;
+    c->ni[0] = c->ni[1] = c->ni[2] = 0;
+    c->cur = 0;
+    c->done = 0;
+}
+
+// First block time after the initial steps: min over particles of t + dt.
+__global__ void __launch_bounds__(256) kb_first_min(const double *__restrict__ t,
+                                                    const double *__restrict__ dt, int n,
+                                                    StepCtl *c)
+{
+    double lo = __builtin_inf();
+    for (int j = blockIdx.x * blockDim.x + threadIdx.x; j < n; j += gridDim.x * blockDim.x)
+        lo = fmin(lo, t[j] + dt[j]);
+    fold_min(lo, &c->tmin[0]);
+}
+
+// Step k, parity par = k & 1.  Ends the run when the system time has reached
+// t_end (ph4's evolve_model loop runs while system_time < t_end,
+// src/amuse_ph4/interface.cc:673-674) or the block budget is spent; every
+// thread reads the same system_time, t_end and budget, which no kernel of
+// this launch writes.  Otherwise the block: particles with t + dt equal to
+// the block time tmin[par], appended to ni[par] in any order (each i's
+// force and correction are independent of the order) and predicted to it
+// into the i-block rows; every j-slot is predicted to it as well (the
+// library's predictor at set_ti); the other particles' t + dt, unchanged by
+// this step, go into the next step's minimum tmin[par ^ 1].
+__global__ void __launch_bounds__(256) kb_select(State s, JStore js, IBlock b, StepCtl *c,
+                                                 int prec, int compat, int par)
 {
     if (c->done) return;
     const bool first = blockIdx.x == 0 && threadIdx.x == 0;
-    // ph4's evolve_model loop runs while system_time < t_end
-    // (src/amuse_ph4/interface.cc:673-674); every thread reads the same
-    // system_time and t_end, which no kernel of this launch writes.
     if (!(c->system_time < c->t_end)) {
         if (first) c->done = 1;  // t_end reached
         return;
@@ -102,26 +143,9 @@ __global__ void __launch_bounds__(256) kb_tmin(const double *__restrict__ t,
         if (first) c->done = 2;  // block budget spent
         return;
     }
-    if (first) c->ni = 0;
+    const double tnext = __longlong_as_double(static_cast<long long>(c->tmin[par]));
+    if (first) c->tnext = tnext;
     double lo = __builtin_inf();
-    for (int j = blockIdx.x * blockDim.x + threadIdx.x; j < n; j += gridDim.x * blockDim.x)
-        lo = fmin(lo, t[j] + dt[j]);
-#pragma unroll
-    for (int off = 32; off >= 1; off >>= 1) lo = fmin(lo, __shfl_xor(lo, off));
-    if ((threadIdx.x & 63) == 0 && lo < __builtin_inf())
-        atomicMin(&c->tmin, static_cast<unsigned long long>(__double_as_longlong(lo)));
-}
-
-// The block: particles with t + dt equal to the next block time, appended in
-// any order (each i's force and correction are independent of the order),
-// predicted to it into the i-block rows; every j-slot is predicted to it as
-// well (the library's predictor at set_ti).
-__global__ void __launch_bounds__(256) kb_select(State s, JStore js, IBlock b, StepCtl *c,
-                                                 int prec, int compat)
-{
-    if (c->done) return;
-    const double tnext = __longlong_as_double(static_cast<long long>(c->tmin));
-    if (blockIdx.x == 0 && threadIdx.x == 0) c->tnext = tnext;
     for (int j = blockIdx.x * blockDim.x + threadIdx.x; j < s.n; j += gridDim.x * blockDim.x) {
         {
             double x[3], v[3], a2[3], j6[3];
@@ -134,8 +158,12 @@ __global__ void __launch_bounds__(256) kb_select(State s, JStore js, IBlock b, S
                                                    false);
             g6k::write_pred(js, j, prec, js.id[j], pr);
         }
-        if (s.t[j] + s.dt[j] != tnext) continue;
-        const int q = atomicAdd(&c->ni, 1);
+        const double tj = s.t[j] + s.dt[j];
+        if (tj != tnext) {
+            lo = fmin(lo, tj);
+            continue;
+        }
+        const int q = atomicAdd(&c->ni[par], 1);
         s.list[q] = j;
         double x[3], v[3], a[3], jk[3], xp[3], vp[3];
 #pragma unroll
@@ -150,6 +178,7 @@ __global__ void __launch_bounds__(256) kb_select(State s, JStore js, IBlock b, S
         }
         b.id[q] = s.id[j];
     }
+    fold_min(lo, &c->tmin[par ^ 1]);
 }
 
 // Every particle as the i-block, predicted to the system time (energy and
@@ -159,7 +188,7 @@ __global__ void __launch_bounds__(256) kb_select_all(State s, IBlock b, StepCtl 
     const double t0 = c->system_time;
     for (int j = blockIdx.x * blockDim.x + threadIdx.x; j < s.n; j += gridDim.x * blockDim.x) {
         if (j == 0) {
-            c->ni = s.n;
+            c->ni[2] = s.n;
             c->done = 0;
         }
         s.list[j] = j;
@@ -221,54 +250,119 @@ __global__ void __launch_bounds__(256) kb_take_all(State s, IBlock b)
     }
 }
 
-__global__ void __launch_bounds__(256) kb_correct(State s, JStore js, IBlock b, StepCtl *c,
-                                                  double eta)
+// Corrects particle list[q] with the forces f (acc3 jerk3 pot), reloads it
+// into the j-store and returns its new t + dt.
+__device__ __forceinline__ double correct_one(const State &s, const JStore &js, const IBlock &b,
+                                              int q, double tnext, double eta, const double *f)
+{
+    const int j = s.list[q];
+    double oa[3], oj[3], ia[3], ij[3], ip[3], iv[3];
+#pragma unroll
+    for (int k = 0; k < 3; k++) {
+        oa[k] = s.a[k][j];
+        oj[k] = s.j[k][j];
+        ia[k] = f[k];
+        ij[k] = f[3 + k];
+        ip[k] = b.x[k][q];
+        iv[k] = b.v[k][q];
+    }
+    const double newstep = h4::correct_ph4(tnext, s.t[j], s.dt[j], eta, oa, oj, ia, ij, ip, iv);
+#pragma unroll
+    for (int k = 0; k < 3; k++) {
+        s.x[k][j] = ip[k];
+        s.v[k][j] = iv[k];
+        s.a[k][j] = ia[k];
+        s.j[k][j] = ij[k];
+    }
+    s.pot[j] = f[6];
+    s.t[j] = tnext;
+    s.dt[j] = newstep;
+    load_j(s, js, j, 0);
+    return s.t[j] + s.dt[j];
+}
+
+// Forces of block row q folded from the j-chunk partials (the order and
+// arithmetic of the library's reduce kernel: lane form, or lane `lane` of the
+// wave form followed by the butterfly; the result is valid in lane 0).
+template <bool MX>
+__device__ __forceinline__ void fold_forces(const double *part, int nchunk, int ni, int q,
+                                            bool wave, double *f)
+{
+    const int lane = threadIdx.x & 63;
+    const int first = wave ? lane : 0, step = wave ? 64 : 1;
+    int addr;
+    if constexpr (MX) {
+        float h[7], l[7], best;
+        g6k::reduce_i_mx(part, nchunk, ni, q, first, step, h, l, best, addr, 0);
+        if (wave) g6k::butterfly_mx(h, l, best, addr);
+#pragma unroll
+        for (int k = 0; k < 7; k++) f[k] = double(h[k]) + double(l[k]);
+    } else {
+        double h[7], l[7], best;
+        g6k::reduce_i_f64(part, nchunk, ni, q, first, step, h, l, best, addr, 0);
+        if (wave) g6k::butterfly_f64(h, l, best, addr);
+#pragma unroll
+        for (int k = 0; k < 7; k++) f[k] = h[k] + l[k];
+    }
+}
+
+// Step k, parity par: the block's forces (from b.out, or folded here from
+// the partials when the plan split j over workgroups), ph4's corrector, the
+// new t + dt into tmin[par ^ 1]; the first thread advances the counters and
+// clears ni[par ^ 1] for the next step's selection.
+template <bool MX>
+__global__ void __launch_bounds__(256) kb_correct(State s, JStore js, IBlock b,
+                                                  const double *__restrict__ part, StepCtl *c,
+                                                  double eta, int par)
 {
     if (c->done) return;
     const double tnext = c->tnext;
-    const int ni = c->ni;
+    const int ni = c->ni[par];
     const int S = b.cap;
     if (blockIdx.x == 0 && threadIdx.x == 0) {
         c->blocks += 1;
         c->psteps += ni;
         c->system_time = tnext;
         if (c->budget > 0) c->budget -= 1;
+        c->ni[par ^ 1] = 0;
+        c->cur = par ^ 1;
     }
-    for (int q = blockIdx.x * blockDim.x + threadIdx.x; q < ni; q += gridDim.x * blockDim.x) {
-        const int j = s.list[q];
-        double oa[3], oj[3], ia[3], ij[3], ip[3], iv[3];
-#pragma unroll
-        for (int k = 0; k < 3; k++) {
-            oa[k] = s.a[k][j];
-            oj[k] = s.j[k][j];
-            ia[k] = b.out[k * S + q];
-            ij[k] = b.out[(3 + k) * S + q];
-            ip[k] = b.x[k][q];
-            iv[k] = b.v[k][q];
+    const g6k::StepPlan sp = g6k::plan_step(ni, s.n);
+    double lo = __builtin_inf();
+    double f[7];
+    if (sp.nchunk > 1 && sp.nchunk >= g6k::kReduceWaveMin && ni <= g6k::kReduceWaveMaxNi) {
+        // one wave per row; the loop bound is wave-uniform
+        for (int q = blockIdx.x * 4 + int(threadIdx.x >> 6); q < ni; q += gridDim.x * 4) {
+            fold_forces<MX>(part, sp.nchunk, ni, q, true, f);
+            if ((threadIdx.x & 63) == 0) lo = fmin(lo, correct_one(s, js, b, q, tnext, eta, f));
         }
-        const double newstep =
-            h4::correct_ph4(tnext, s.t[j], s.dt[j], eta, oa, oj, ia, ij, ip, iv);
+    } else {
+        for (int q = blockIdx.x * blockDim.x + threadIdx.x; q < ni; q += gridDim.x * blockDim.x) {
+            if (sp.nchunk > 1) {
+                fold_forces<MX>(part, sp.nchunk, ni, q, false, f);
+            } else {
 #pragma unroll
-        for (int k = 0; k < 3; k++) {
-            s.x[k][j] = ip[k];
-            s.v[k][j] = iv[k];
-            s.a[k][j] = ia[k];
-            s.j[k][j] = ij[k];
+                for (int k = 0; k < 7; k++) f[k] = b.out[k * S + q];
+            }
+            lo = fmin(lo, correct_one(s, js, b, q, tnext, eta, f));
         }
-        s.pot[j] = b.out[6 * S + q];
-        s.t[j] = tnext;
-        s.dt[j] = newstep;
-        load_j(s, js, j, 0);
     }
+    fold_min(lo, &c->tmin[par ^ 1]);
 }
 
+// Arms a run: end time and block budget; the pending block time moves to
+// slot 0, where the captured sequence (which starts at parity 0) reads it.
 __global__ void kb_arm(StepCtl *c, double t_end, long long budget)
 {
     c->t_end = t_end;
     c->budget = budget;
     c->done = 0;
-    c->tmin = 0x7ff0000000000000ull;  // +inf
-    c->ni = 0;
+    if (c->cur == 1) {
+        c->tmin[0] = c->tmin[1];
+        c->tmin[1] = kInfBits;
+        c->ni[0] = 0;
+        c->cur = 0;
+    }
 }
 
 // ---------------------------------------------------------------------------
@@ -276,16 +370,23 @@ __global__ void kb_arm(StepCtl *c, double t_end, long long budget)
 // ---------------------------------------------------------------------------
 constexpr int kGrid = 1024;  // workgroups of the per-particle kernels
 
-int enqueue_step(Driver *D)
+int enqueue_step(Driver *D, int par)
 {
     const int n = D->st.n;
     const int g = std::min(nblk(n, 256), kGrid);
-    hipLaunchKernelGGL(kb_tmin, dim3(g), dim3(256), 0, D->stream, D->st.t, D->st.dt, n, D->ctl);
     hipLaunchKernelGGL(kb_select, dim3(g), dim3(256), 0, D->stream, D->st, D->js, D->ib, D->ctl,
-                       D->prec, D->compat);
-    DCK(g6k::launch_step_force(D->js, D->ib, D->part, n, D->eps2, D->prec, D->ctl, D->stream));
-    hipLaunchKernelGGL(kb_correct, dim3(g), dim3(256), 0, D->stream, D->st, D->js, D->ib, D->ctl,
-                       D->eta);
+                       D->prec, D->compat, par);
+    DCK(g6k::launch_step_force(D->js, D->ib, D->part, n, D->eps2, D->prec, D->ctl, par, par,
+                               false, D->stream));
+    // enough waves for one block row each when the partials are folded per
+    // wave (up to n rows, four waves per workgroup)
+    const int gc = std::min(g6k::kStepGrid, std::max(g, nblk(n, 4)));
+    if (D->prec == G6X_MIXED)
+        hipLaunchKernelGGL(kb_correct<true>, dim3(gc), dim3(256), 0, D->stream, D->st, D->js,
+                           D->ib, D->part, D->ctl, D->eta, par);
+    else
+        hipLaunchKernelGGL(kb_correct<false>, dim3(gc), dim3(256), 0, D->stream, D->st, D->js,
+                           D->ib, D->part, D->ctl, D->eta, par);
     DCK(hipGetLastError());
     return 0;
 }
@@ -297,7 +398,8 @@ int full_pass(Driver *D, int prec)
     const int g = std::min(nblk(n, 256), kGrid);
     hipLaunchKernelGGL(kb_select_all, dim3(g), dim3(256), 0, D->stream, D->st, D->ib, D->ctl);
     DCK(g6k::launch_step_predict(D->js, n, D->ctl, 1, prec, D->compat, D->stream));
-    DCK(g6k::launch_step_force(D->js, D->ib, D->part, n, D->eps2, prec, D->ctl, D->stream));
+    DCK(g6k::launch_step_force(D->js, D->ib, D->part, n, D->eps2, prec, D->ctl, 2, -1, true,
+                               D->stream));
     DCK(hipGetLastError());
     return 0;
 }
@@ -385,7 +487,7 @@ int setup(Driver *D, int n, const int *ids, const double *m, const double *x, co
     DCK(hipMemcpyAsync(s.dt, zeros.data(), nd * 8, hipMemcpyHostToDevice, D->stream));
     DCK(hipMemcpyAsync(s.pot, zeros.data(), nd * 8, hipMemcpyHostToDevice, D->stream));
     StepCtl c0{};
-    c0.tmin = 0;
+    c0.tmin[0] = c0.tmin[1] = g6b::kInfBits;
     c0.budget = -1;
     c0.t_end = 0.0;
     c0.system_time = 0.0;
@@ -430,6 +532,9 @@ int commit(Driver *D)
     // dtj of the j-store follows; acc/jerk stay zero there unless reloaded
     hipLaunchKernelGGL(kb_load_all, dim3(g), dim3(256), 0, D->stream, s, D->js,
                        D->ph4_initial_load ? 1 : 0);
+    // the first block time
+    hipLaunchKernelGGL(kb_ctl_reset, dim3(1), dim3(1), 0, D->stream, D->ctl);
+    hipLaunchKernelGGL(kb_first_min, dim3(g), dim3(256), 0, D->stream, s.t, s.dt, n, D->ctl);
     DCK(hipGetLastError());
     DCK(hipStreamSynchronize(D->stream));
     return 0;
@@ -442,7 +547,7 @@ int capture(Driver *D, int steps)
     hipGraph_t g;
     DCK(hipStreamBeginCapture(D->stream, hipStreamCaptureModeThreadLocal));
     for (int k = 0; k < steps; k++)
-        if (enqueue_step(D)) {
+        if (enqueue_step(D, k & 1)) {
             hipStreamEndCapture(D->stream, &g);
             return G6X_ERR_HIP;
         }
@@ -496,7 +601,9 @@ G6D_API long long g6d_run(void *h, double t_end, long long max_blocks, int steps
 {
     if (!h) return G6X_ERR_ARG;
     Driver *D = static_cast<Driver *>(h);
-    if (steps_per_graph < 1) steps_per_graph = 1;
+    // an even number of steps per graph: a replay starts at parity 0
+    if (steps_per_graph < 2) steps_per_graph = 2;
+    steps_per_graph += steps_per_graph & 1;
     if (!D->graph || D->graph_steps != steps_per_graph || D->graph_prec != D->prec)
         if (g6b::capture(D, steps_per_graph)) return G6X_ERR_HIP;
     if (g6b::read_ctl(D)) return G6X_ERR_HIP;

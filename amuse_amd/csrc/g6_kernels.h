@@ -83,20 +83,27 @@ struct ForceCfg {
 // control block and size their work themselves; one captured launch
 // sequence then serves every step.
 //
-// Protocol of one step: the next-time kernel resets ni and folds min(t + dt)
-// into tmin; the selection kernel turns tmin into tnext, appends the block to
-// ni; the force kernel resets tmin for the next step (nothing reads it after
-// the selection); the correction kernel advances the counters.  Each field
-// is reset by a kernel that no other kernel of the same launch reads it in.
+// Protocol of one step k (three kernels; par = k & 1 is a launch argument,
+// fixed in the captured sequence, so a captured graph holds an even number
+// of steps): the selection kernel ends the run (system_time >= t_end, or the
+// block budget spent) or turns tmin[par] into tnext, appends the block to
+// ni[par] and folds min(t + dt) of the other particles into tmin[par ^ 1];
+// the force kernel resets tmin[par] (consumed); the correction kernel folds
+// the force partials, corrects, folds the new t + dt into tmin[par ^ 1],
+// resets ni[par ^ 1] for the next step and advances the counters and cur.
+// Each field is reset by a kernel that no other kernel of the same launch
+// reads it in.  ni[2] serves full passes (energies, initial forces), which
+// leave the step fields alone.
 struct StepCtl {
-    unsigned long long tmin;  // bits of min(t + dt) (positive doubles order as integers)
-    int ni;                   // active particles of the current step
-    int done;                 // 1: t_end reached, 2: block budget spent
-    long long blocks, psteps; // block and particle steps taken
-    long long budget;         // block steps left in the current run (< 0: no limit)
+    unsigned long long tmin[2]; // bits of min(t + dt) (positive doubles order as integers)
+    int ni[3];                  // active particles: steps of parity 0 and 1, full passes
+    int cur;                    // parity of the next step (tmin[cur] holds its time)
+    int done;                   // 1: t_end reached, 2: block budget spent
+    long long blocks, psteps;   // block and particle steps taken
+    long long budget;           // block steps left in the current run (< 0: no limit)
     double t_end;
     double system_time;
-    double tnext;             // block time of the current step
+    double tnext;               // block time of the current step
 };
 
 struct StepPlan {
@@ -158,10 +165,14 @@ __host__ __device__ inline StepPlan plan_step(int ni, int nj)
 hipError_t launch_step_predict(const JStore &s, int nj, const StepCtl *c, int at_system,
                                int prec, int compat, hipStream_t st);
 // Forces (acc, jerk, pot; no nn) of the step's i-block (b rows 0..ni-1,
-// ni = c->ni) against j-slots [0, nj) into b.out, with partials in part
-// (capacity kStepPartBudget * fields doubles) reduced by a second kernel.
+// ni = c->ni[ni_slot]) against j-slots [0, nj): into b.out when the plan has
+// one j-chunk, else as partials in part (capacity kStepPartBudget * fields
+// doubles), which a second kernel reduces into b.out when with_reduce (the
+// block step's correction kernel folds them itself).  tmin_slot >= 0: the
+// force kernel resets c->tmin[tmin_slot] (see StepCtl).
 hipError_t launch_step_force(const JStore &s, const IBlock &b, double *part, int nj,
-                             double eps2, int prec, StepCtl *c, hipStream_t st);
+                             double eps2, int prec, StepCtl *c, int ni_slot, int tmin_slot,
+                             bool with_reduce, hipStream_t st);
 
 size_t jstore_bytes(int cap);
 void jstore_carve(JStore &s, void *base, int cap);

@@ -24,6 +24,7 @@
 // j-split.
 #include "g6_kernels.h"
 #include "g6_predict.h"
+#include "g6_reduce.h"
 
 #include <algorithm>
 #include <cfloat>
@@ -230,14 +231,6 @@ struct DDAcc {
     double h[7], l[7];
 };
 
-__device__ __forceinline__ void two_sum_add_d(double &hi, double &lo, double x)
-{
-    const double s = hi + x;
-    const double bp = s - hi;
-    const double err = (hi - (s - bp)) + (x - bp);
-    hi = s;
-    lo += err;
-}
 
 __device__ __forceinline__ void flush_f64(AccD &a, DDAcc &d)
 {
@@ -284,21 +277,6 @@ __device__ __forceinline__ void pair_f64(const ID &I, AccD &a, double xj, double
     a.jz = __builtin_fma(mi3, __builtin_fma(c, dz, dvz), a.jz);
 }
 
-// Lexicographic (r2, slot) minimum; an empty key has r2 = +inf, slot = -1
-// (as unsigned, the largest).
-__device__ __forceinline__ void nn_merge(double &best, int &addr, double ob, int oa)
-{
-    const bool take = (ob < best) || (ob == best && unsigned(oa) < unsigned(addr));
-    best = take ? ob : best;
-    addr = take ? oa : addr;
-}
-
-__device__ __forceinline__ void nn_merge_f(float &best, int &addr, float ob, int oa)
-{
-    const bool take = (ob < best) || (ob == best && unsigned(oa) < unsigned(addr));
-    best = take ? ob : best;
-    addr = take ? oa : addr;
-}
 
 struct FParams {
     const double *ix, *iy, *iz, *ivx, *ivy, *ivz;
@@ -493,14 +471,6 @@ struct DSAcc {
     float h[7], l[7];
 };
 
-__device__ __forceinline__ void two_sum_add(float &hi, float &lo, float x)
-{
-    const float s = hi + x;
-    const float bp = s - hi;
-    const float err = (hi - (s - bp)) + (x - bp);
-    hi = s;
-    lo += err;
-}
 
 // Difference vector and r^2 of one pair, mixed path: dx from double-single
 // positions, r^2 = (dx dx + dy dy) + dz dz with FMAs.  The self pair gets
@@ -860,99 +830,6 @@ __global__ void __launch_bounds__(kWG) k_force_mx_s(FParams p)
     }
 }
 
-// ------------------------------------------------------------------------
-// Ordered reduction of the per-chunk partials.  Partials are
-// [chunk][field][PS], results [7][S] (+ nn).  Two forms: one lane folds all
-// chunks of its i in chunk order; or one wave per i, lane L folding chunks
-// L, L + 64, ... followed by a fixed butterfly over the lanes (for small
-// i-blocks with many chunks, where the serial loop is a latency chain).
-// Either way the result does not depend on timing.
-// ------------------------------------------------------------------------
-__device__ __forceinline__ void fold_chunk_f64(const double *__restrict__ part, int ch, int PS,
-                                               int i, double *h, double *l, double &best,
-                                               int &addr, int want_nn)
-{
-    const double *r = part + size_t(ch) * kPartFieldsD * PS;
-#pragma unroll
-    for (int k = 0; k < 7; k++) {
-        two_sum_add_d(h[k], l[k], r[(2 * k) * PS + i]);
-        l[k] += r[(2 * k + 1) * PS + i];
-    }
-    if (want_nn) nn_merge(best, addr, r[14 * PS + i], int(r[15 * PS + i]));
-}
-
-__device__ __forceinline__ void fold_chunk_mx(const double *__restrict__ part, int ch, int PS,
-                                              int i, float *h, float *l, float &best, int &addr,
-                                              int want_nn)
-{
-    const float2 *r = reinterpret_cast<const float2 *>(part + size_t(ch) * kPartFields * PS);
-#pragma unroll
-    for (int k = 0; k < 7; k++) {
-        const float2 v = r[k * PS + i];
-        two_sum_add(h[k], l[k], v.x);
-        l[k] += v.y;
-    }
-    if (want_nn) nn_merge_f(best, addr, r[7 * PS + i].x, __float_as_int(r[8 * PS + i].x));
-}
-
-// lane form (first == 0, step == 1) or one lane of the wave form
-// (first = lane, step = 64, followed by butterfly_*)
-__device__ __forceinline__ void reduce_i_f64(const double *__restrict__ part, int nchunk, int PS,
-                                             int i, int first, int step, double *h, double *l,
-                                             double &best, int &addr, int want_nn)
-{
-#pragma unroll
-    for (int k = 0; k < 7; k++) { h[k] = 0.0; l[k] = 0.0; }
-    best = __builtin_inf();
-    addr = -1;
-    for (int ch = first; ch < nchunk; ch += step)
-        fold_chunk_f64(part, ch, PS, i, h, l, best, addr, want_nn);
-}
-
-__device__ __forceinline__ void reduce_i_mx(const double *__restrict__ part, int nchunk, int PS,
-                                            int i, int first, int step, float *h, float *l,
-                                            float &best, int &addr, int want_nn)
-{
-#pragma unroll
-    for (int k = 0; k < 7; k++) { h[k] = 0.0f; l[k] = 0.0f; }
-    best = __builtin_inff();
-    addr = -1;
-    for (int ch = first; ch < nchunk; ch += step)
-        fold_chunk_mx(part, ch, PS, i, h, l, best, addr, want_nn);
-}
-
-__device__ __forceinline__ void butterfly_f64(double *h, double *l, double &best, int &addr)
-{
-#pragma unroll
-    for (int off = 32; off >= 1; off >>= 1) {
-#pragma unroll
-        for (int k = 0; k < 7; k++) {
-            const double oh = __shfl_xor(h[k], off), ol = __shfl_xor(l[k], off);
-            two_sum_add_d(h[k], l[k], oh);
-            l[k] += ol;
-        }
-        const double ob = __shfl_xor(best, off);
-        const int oa = __shfl_xor(addr, off);
-        nn_merge(best, addr, ob, oa);
-    }
-}
-
-__device__ __forceinline__ void butterfly_mx(float *h, float *l, float &best, int &addr)
-{
-#pragma unroll
-    for (int off = 32; off >= 1; off >>= 1) {
-#pragma unroll
-        for (int k = 0; k < 7; k++) {
-            const float oh = __shfl_xor(h[k], off), ol = __shfl_xor(l[k], off);
-            two_sum_add(h[k], l[k], oh);
-            l[k] += ol;
-        }
-        const float ob = __shfl_xor(best, off);
-        const int oa = __shfl_xor(addr, off);
-        nn_merge_f(best, addr, ob, oa);
-    }
-}
-
 template <class T>
 __device__ __forceinline__ void store_reduced(const T *h, const T *l, int addr, int S, int i,
                                               const int *__restrict__ jid,
@@ -1029,12 +906,13 @@ __global__ void __launch_bounds__(256) kd_predict(JStore s, int nj, const StepCt
 }
 
 template <bool MX>
-__global__ void __launch_bounds__(kWG) kd_force(FParams p, StepCtl *c)
+__global__ void __launch_bounds__(kWG) kd_force(FParams p, StepCtl *c, int ni_slot, int tmin_slot)
 {
     if (c->done) return;
-    // the block is selected: the next step's minimum starts from +inf
-    if (blockIdx.x == 0 && threadIdx.x == 0) c->tmin = 0x7ff0000000000000ull;
-    const int ni = c->ni;
+    // the block is selected: its time slot is free for the step after next
+    if (tmin_slot >= 0 && blockIdx.x == 0 && threadIdx.x == 0)
+        c->tmin[tmin_slot] = 0x7ff0000000000000ull;
+    const int ni = c->ni[ni_slot];
     if (ni <= 0) return;
     const StepPlan sp = plan_step(ni, p.nj);
     p.ni = ni;
@@ -1070,10 +948,10 @@ template <bool MX>
 __global__ void __launch_bounds__(256) kd_reduce(const double *__restrict__ part,
                                                  const StepCtl *c, int nj, int S,
                                                  const int *__restrict__ jid,
-                                                 double *__restrict__ out)
+                                                 double *__restrict__ out, int ni_slot)
 {
     if (c->done) return;
-    const int ni = c->ni;
+    const int ni = c->ni[ni_slot];
     if (ni <= 0) return;
     const StepPlan sp = plan_step(ni, nj);
     if (sp.nchunk <= 1) return;
@@ -1304,7 +1182,8 @@ hipError_t launch_step_predict(const JStore &s, int nj, const StepCtl *c, int at
 }
 
 hipError_t launch_step_force(const JStore &s, const IBlock &b, double *part, int nj,
-                             double eps2, int prec, StepCtl *c, hipStream_t st)
+                             double eps2, int prec, StepCtl *c, int ni_slot, int tmin_slot,
+                             bool with_reduce, hipStream_t st)
 {
     if (nj <= 0) return hipSuccess;
     FParams fp{};
@@ -1330,13 +1209,17 @@ hipError_t launch_step_force(const JStore &s, const IBlock &b, double *part, int
     const int gf = std::min(kStepGrid, std::max(64, nblocks(nj, 4)));
     const int gr = std::min(kStepGrid, std::max(1, nblocks(nj, 64)));
     if (prec == kF64) {
-        hipLaunchKernelGGL((kd_force<false>), dim3(gf), dim3(kWG), 0, st, fp, c);
-        hipLaunchKernelGGL((kd_reduce<false>), dim3(gr), dim3(256), 0, st, part, c, nj, b.cap,
-                           s.pid, b.out);
+        hipLaunchKernelGGL((kd_force<false>), dim3(gf), dim3(kWG), 0, st, fp, c, ni_slot,
+                           tmin_slot);
+        if (with_reduce)
+            hipLaunchKernelGGL((kd_reduce<false>), dim3(gr), dim3(256), 0, st, part, c, nj,
+                               b.cap, s.pid, b.out, ni_slot);
     } else {
-        hipLaunchKernelGGL((kd_force<true>), dim3(gf), dim3(kWG), 0, st, fp, c);
-        hipLaunchKernelGGL((kd_reduce<true>), dim3(gr), dim3(256), 0, st, part, c, nj, b.cap,
-                           s.pid, b.out);
+        hipLaunchKernelGGL((kd_force<true>), dim3(gf), dim3(kWG), 0, st, fp, c, ni_slot,
+                           tmin_slot);
+        if (with_reduce)
+            hipLaunchKernelGGL((kd_reduce<true>), dim3(gr), dim3(256), 0, st, part, c, nj,
+                               b.cap, s.pid, b.out, ni_slot);
     }
     return hipGetLastError();
 }
