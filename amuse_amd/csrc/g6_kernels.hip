@@ -537,6 +537,8 @@ __device__ __forceinline__ void flush_mx(AccF &a, DSAcc &d)
 #define G6_MX_FLUSH 64
 #endif
 constexpr int kFlush = G6_MX_FLUSH;
+// the final nearest-neighbour search gives one lane to each j of a block
+static_assert(kFlush <= 64, "a block of a lane's j must fit one wave");
 
 // One (i-block bx, j-chunk by) work item; the kernel below runs one item
 // per workgroup, kd_* below loops over items whose count lives on the device.
@@ -610,19 +612,33 @@ __device__ __forceinline__ void force_mx_item(const FParams &p, TileF &tile, int
     if (NN) {
         // The slot: the first j of the best block, in the lane's stride,
         // with exactly that r^2 (recomputed from the same arrays the tiles
-        // were loaded from, so bit-identical).
+        // were loaded from, so bit-identical).  The wave searches one lane's
+        // block at a time: lane k takes the block's k-th j (a block holds at
+        // most kFlush of a lane's j), so each block is read once, coalesced,
+        // and the first match is the lowest set bit of the ballot.
+        const int lane = threadIdx.x & 63;
 #pragma unroll
         for (int q = 0; q < IPL; q++) {
-            if (!(A[q].best < __builtin_inff())) continue;
-            for (int j = A[q].blo; j < A[q].bhi; j += NJS) {
-                const float4 a = make_float4(p.hx[j], p.hy[j], p.hz[j], 0.0f);
-                const float4 b = make_float4(p.lx[j], p.ly[j], p.lz[j], 0.0f);
-                const float4 c = make_float4(0.0f, 0.0f, __int_as_float(p.jid[j]), 0.0f);
-                float dx, dy, dz;
-                if (pair_r2_mx(I[q], a, b, c, dx, dy, dz) == A[q].best) {
-                    A[q].addr = j;
-                    break;
+            for (int L = 0; L < 64; L++) {
+                const float best = __shfl(A[q].best, L);
+                if (!(best < __builtin_inff())) continue;  // wave-uniform
+                const int blo = __shfl(A[q].blo, L), bhi = __shfl(A[q].bhi, L);
+                IF ii{};
+                ii.hx = __shfl(I[q].hx, L); ii.hy = __shfl(I[q].hy, L);
+                ii.hz = __shfl(I[q].hz, L); ii.lx = __shfl(I[q].lx, L);
+                ii.ly = __shfl(I[q].ly, L); ii.lz = __shfl(I[q].lz, L);
+                ii.id = __shfl(I[q].id, L);
+                const int j = blo + lane * NJS;
+                bool hit = false;
+                if (j < bhi) {
+                    const float4 a = make_float4(p.hx[j], p.hy[j], p.hz[j], 0.0f);
+                    const float4 b = make_float4(p.lx[j], p.ly[j], p.lz[j], 0.0f);
+                    const float4 c = make_float4(0.0f, 0.0f, __int_as_float(p.jid[j]), 0.0f);
+                    float dx, dy, dz;
+                    hit = pair_r2_mx(ii, a, b, c, dx, dy, dz) == best;
                 }
+                const unsigned long long mask = __ballot(hit);
+                if (lane == L && mask) A[q].addr = blo + (__ffsll((long long)mask) - 1) * NJS;
             }
         }
     }
