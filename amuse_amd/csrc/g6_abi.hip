@@ -36,7 +36,7 @@ using g6k::JStore;
 
 constexpr int kMaxPipes = 1 << 22;
 constexpr int kMaxSlots = 1 << 27;
-constexpr long long kPartBudget = 1ll << 21;  // nchunk * ni limit for partials
+constexpr long long kPartBudget = 1ll << 24;  // nchunk * ni limit for partials (<= 2 GB)
 
 struct TuneEntry {
     int ni_lo, ni_hi;  // bucket [lo, hi]
@@ -251,20 +251,23 @@ int grow_iblock(int need)
     return 0;
 }
 
-int ensure_partials(int nchunk)
+// Partials of a launch are packed [nchunk][fields][ni]; the buffer grows to
+// the largest nchunk * ni seen (at most kPartBudget records).
+int ensure_partials(int nchunk, int ni)
 {
-    if (nchunk <= 1 || nchunk <= U.ib.part_chunks) return 0;
+    const long long need = (long long)nchunk * ni;
+    if (nchunk <= 1 || need <= U.ib.part_entries) return 0;
     HIPCHK(hipStreamSynchronize(U.stream), "partials sync");
     if (U.ib.part_base) hipFree(U.ib.part_base);
     U.ib.part_base = nullptr;
     U.ib.part = nullptr;
-    U.ib.part_chunks = 0;
+    U.ib.part_entries = 0;
     void *p = nullptr;
-    hipError_t e = hipMalloc(&p, sizeof(double) * g6k::kPartFieldsD * size_t(U.ib.cap) * size_t(nchunk));
+    hipError_t e = hipMalloc(&p, sizeof(double) * g6k::kPartFieldsD * size_t(need));
     if (e != hipSuccess) return fail(G6X_ERR_NOMEM, "hipMalloc partials", e);
     U.ib.part_base = p;
     U.ib.part = static_cast<double *>(p);
-    U.ib.part_chunks = nchunk;
+    U.ib.part_entries = need;
     return 0;
 }
 
@@ -351,7 +354,15 @@ ForceCfg choose_cfg(int ni, int nj)
     const int ib = (g6k::kWG / njs) * ipl;
     const int iblocks = (ni + ib - 1) / ib;
     const int max_chunks = std::max(1, (nj + g6k::kTJ - 1) / g6k::kTJ);
-    if (nchunk <= 0) nchunk = std::max(1, (2048 + iblocks - 1) / iblocks);
+    if (nchunk <= 0) {
+        // ~8 workgroups per CU at least; for long j-ranges up to ~128, so a
+        // launch is many rounds of workgroups and the last round's imbalance
+        // is small (at ni = nj = 2^20, two rounds lost 5 % against 32), with
+        // every chunk still >= 16 tiles
+        const int base = (2048 + iblocks - 1) / iblocks;
+        const int deep = std::min((32768 + iblocks - 1) / iblocks, nj / (16 * g6k::kTJ));
+        nchunk = std::max(1, std::max(base, deep));
+    }
     nchunk = std::min(nchunk, max_chunks);
     while (nchunk > 1 && (long long)nchunk * ni > kPartBudget) nchunk /= 2;
     int jchunk = (nj + nchunk - 1) / nchunk;
@@ -370,7 +381,7 @@ int run_force(int ni, int nj, double eps2, bool want_nn)
 {
     if (ni <= 0) { U.last_ni = 0; return 0; }
     ForceCfg cfg = choose_cfg(ni, std::max(nj, 1));
-    int rc = ensure_partials(cfg.nchunk);
+    int rc = ensure_partials(cfg.nchunk, ni);
     if (rc) return rc;
     Unit::Ev ev;
     timed_begin(ev, 0, double(ni) * double(nj));

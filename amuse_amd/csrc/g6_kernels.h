@@ -60,8 +60,8 @@ struct IBlock {
     int *id = nullptr;
     double *out = nullptr;   // [7][cap]
     int *nn = nullptr;       // [cap]
-    double *part = nullptr;  // [nchunk_max][9][cap]
-    int part_chunks = 0;
+    double *part = nullptr;  // [nchunk][fields][ni] of the last launch
+    long long part_entries = 0;  // (chunk, i) records the buffer holds
     void *base = nullptr;
     void *part_base = nullptr;
 };

@@ -540,12 +540,22 @@ constexpr int kFlush = G6_MX_FLUSH;
 // the final nearest-neighbour search gives one lane to each j of a block
 static_assert(kFlush <= 64, "a block of a lane's j must fit one wave");
 
+// Lanes' j per item from which the nearest neighbour is kept as a running
+// minimum with one slot search at the end (below: a compare per pair).
+constexpr int kLongItem = 16384;
+
 // One (i-block bx, j-chunk by) work item; the kernel below runs one item
 // per workgroup, kd_* below loops over items whose count lives on the device.
-template <int NJS, int IPL, bool NN>
+// LONG (with NN): a lane's j in the item number >= kLongItem, so the
+// nearest neighbour is a running minimum per block of kFlush j with the slot
+// found once at the end; otherwise a compare and two selects per pair, as a
+// final search (up to kFlush dependent loads) would dominate a short item.
+// Both give the first slot with the lane's minimal r^2.
+template <int NJS, int IPL, bool NN, bool LONG>
 __device__ __forceinline__ void force_mx_item(const FParams &p, TileF &tile, int bx, int by,
                                        int nchunk)
 {
+    constexpr int NNM = NN ? (LONG ? 2 : 1) : 0;
     constexpr int IBL = kWG / NJS;
     constexpr int IB = IBL * IPL;
 
@@ -591,11 +601,11 @@ __device__ __forceinline__ void force_mx_item(const FParams &p, TileF &tile, int
                 const float4 a = tile.a[jj], b = tile.b[jj], c = tile.c[jj];
 #pragma unroll
                 for (int q = 0; q < IPL; q++)
-                    pair_mx<NN ? 2 : 0>(I[q], A[q], a, b, c, tb + jj, eps2);
+                    pair_mx<NNM>(I[q], A[q], a, b, c, tb + jj, eps2);
             }
 #pragma unroll
             for (int q = 0; q < IPL; q++) flush_mx(A[q], D[q]);
-            if (NN) {
+            if (NNM == 2) {
                 // A block minimum below the best so far (strictly: an equal
                 // r^2 keeps the earlier block) moves the best to this block.
 #pragma unroll
@@ -609,13 +619,13 @@ __device__ __forceinline__ void force_mx_item(const FParams &p, TileF &tile, int
             }
         }
     }
-    if (NN) {
+    if (NNM == 2) {
         // The slot: the first j of the best block, in the lane's stride,
         // with exactly that r^2 (recomputed from the same arrays the tiles
         // were loaded from, so bit-identical).  The wave searches one lane's
-        // block at a time: lane k takes the block's k-th j (a block holds at
-        // most kFlush of a lane's j), so each block is read once, coalesced,
-        // and the first match is the lowest set bit of the ballot.
+        // block at a time, lane k taking the block's k-th j (a block holds
+        // at most kFlush of a lane's j), so each block is read once,
+        // coalesced, and the first match is the lowest set bit of the ballot.
         const int lane = threadIdx.x & 63;
 #pragma unroll
         for (int q = 0; q < IPL; q++) {
@@ -719,7 +729,10 @@ template <int NJS, int IPL, bool NN>
 __global__ void __launch_bounds__(kWG) k_force_mx(FParams p)
 {
     __shared__ TileF tile;
-    force_mx_item<NJS, IPL, NN>(p, tile, blockIdx.x, blockIdx.y, gridDim.y);
+    if (NN && (p.jchunk + NJS - 1) / NJS >= kLongItem)
+        force_mx_item<NJS, IPL, NN, true>(p, tile, blockIdx.x, blockIdx.y, gridDim.y);
+    else
+        force_mx_item<NJS, IPL, NN, false>(p, tile, blockIdx.x, blockIdx.y, gridDim.y);
 }
 
 // ------------------------------------------------------------------------
@@ -940,10 +953,10 @@ __global__ void __launch_bounds__(kWG) kd_force(FParams p, StepCtl *c, int ni_sl
         for (int w = blockIdx.x; w < items; w += gridDim.x) {
             const int bx = w % sp.iblocks, by = w / sp.iblocks;
             switch (sp.shape) {
-            case 0: force_mx_item<1, 2, false>(p, tile, bx, by, sp.nchunk); break;
-            case 1: force_mx_item<4, 1, false>(p, tile, bx, by, sp.nchunk); break;
-            case 2: force_mx_item<16, 1, false>(p, tile, bx, by, sp.nchunk); break;
-            default: force_mx_item<64, 1, false>(p, tile, bx, by, sp.nchunk); break;
+            case 0: force_mx_item<1, 2, false, false>(p, tile, bx, by, sp.nchunk); break;
+            case 1: force_mx_item<4, 1, false, false>(p, tile, bx, by, sp.nchunk); break;
+            case 2: force_mx_item<16, 1, false, false>(p, tile, bx, by, sp.nchunk); break;
+            default: force_mx_item<64, 1, false, false>(p, tile, bx, by, sp.nchunk); break;
             }
         }
     } else {
@@ -1131,7 +1144,7 @@ hipError_t launch_force(const JStore &s, const IBlock &b, int ni, int nj, double
     fp.out = b.out;
     fp.nn = b.nn;
     fp.ostride = b.cap;
-    fp.pstride = b.cap;
+    fp.pstride = ni;  // partials packed: nchunk * ni records
     fp.part = b.part;
     const int ib = (kWG / cfg.njs) * cfg.ipl;
     dim3 grid(nblocks(ni, ib), cfg.nchunk);
@@ -1172,18 +1185,18 @@ hipError_t launch_reduce(const JStore &s, const IBlock &b, int ni, int nchunk, i
     if (nchunk >= kReduceWaveMin && ni <= kReduceWaveMaxNi) {
         if (prec == kF64)
             hipLaunchKernelGGL((k_reduce<false, true>), dim3(nblocks(ni, 4)), dim3(256), 0, st,
-                               b.part, nchunk, ni, S, S, s.pid, b.out, b.nn, int(want_nn));
+                               b.part, nchunk, ni, ni, S, s.pid, b.out, b.nn, int(want_nn));
         else
             hipLaunchKernelGGL((k_reduce<true, true>), dim3(nblocks(ni, 4)), dim3(256), 0, st,
-                               b.part, nchunk, ni, S, S, s.pid, b.out, b.nn, int(want_nn));
+                               b.part, nchunk, ni, ni, S, s.pid, b.out, b.nn, int(want_nn));
         return hipGetLastError();
     }
     if (prec == kF64)
         hipLaunchKernelGGL((k_reduce<false, false>), dim3(nblocks(ni, 256)), dim3(256), 0, st,
-                           b.part, nchunk, ni, S, S, s.pid, b.out, b.nn, int(want_nn));
+                           b.part, nchunk, ni, ni, S, s.pid, b.out, b.nn, int(want_nn));
     else
         hipLaunchKernelGGL((k_reduce<true, false>), dim3(nblocks(ni, 256)), dim3(256), 0, st,
-                           b.part, nchunk, ni, S, S, s.pid, b.out, b.nn, int(want_nn));
+                           b.part, nchunk, ni, ni, S, s.pid, b.out, b.nn, int(want_nn));
     return hipGetLastError();
 }
 

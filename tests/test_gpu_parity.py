@@ -152,6 +152,42 @@ def test_mixed_nn_ties_on_lattice(g6):
         g6.set_precision(0)
 
 
+def test_mixed_nn_ties_long_items(g6):
+    """Items whose lanes see >= 16384 j keep the mixed nearest neighbour as a
+    running minimum per block and search the slot once at the end (the wave
+    takes one lane's block at a time); shorter items compare per pair.  On an
+    integer lattice (every FP32 r^2 exact, six equidistant neighbours per
+    point) in shuffled slot order, both give the reference's first slot."""
+    g = np.stack(np.meshgrid(np.arange(32), np.arange(32), np.arange(40), indexing="ij"),
+                 -1).reshape(-1, 3).astype(np.float64)
+    nj = g.shape[0]
+    rng = np.random.Generator(np.random.PCG64(77))
+    x = g[rng.permutation(nj)]
+    ids = np.arange(1, nj + 1, dtype=np.int32)
+    m = rng.uniform(0.5, 1.5, nj) / nj
+    v = rng.normal(0, 0.1, (nj, 3))
+    z = np.zeros((nj, 3))
+    g6.reset()
+    g6.set_j_particles(np.arange(nj, dtype=np.int32), ids, 0.0, 0.0, m, z, z, z, v, x)
+    g6.set_ti(0.0)
+    sel = rng.choice(nj, 256, replace=False)
+    id_i, xi, vi = ids[sel], x[sel], v[sel]
+    _, _, _, onn, _ = oracle.force(id_i, xi, vi, 0.0, ids, m, x, v)
+    g6.set_precision(1)
+    try:
+        # (njs, ipl, nchunk): lanes' j per item 40960, 40960, 20480 (long);
+        # 10240, 10240, 2560 (short)
+        for njs, ipl, nchunk in [(1, 2, 1), (1, 1, 1), (1, 2, 2), (1, 2, 4), (4, 1, 1),
+                                 (16, 1, 1)]:
+            print_case(f"mixed nn lattice nj={nj} njs={njs} ipl={ipl} nchunk={nchunk}")
+            g6.set_tiles(njs, ipl, nchunk, 0)
+            nn = g6.calc(nj, id_i, xi, vi, 0.0)[3]
+            assert np.array_equal(nn, onn), (njs, ipl, nchunk)
+    finally:
+        g6.set_tiles(0, 0)
+        g6.set_precision(0)
+
+
 def test_tile_shapes_agree(g6):
     s = plummer_state(3000, 22)
     upload(g6, s["x"], s["v"], s["a2"], s["j6"], s["k18"], s["m"], s["tj"], s["ids"])
