@@ -159,22 +159,22 @@ void timed_begin(Unit::Ev &ev, int kind, double pairs)
     if (!U.timing) return;
     ev.a = pool_event();
     ev.b = pool_event();
-    if (ev.a) hipEventRecord(ev.a, U.stream);
+    if (ev.a) (void)hipEventRecord(ev.a, U.stream);
 }
 
 void timed_end(Unit::Ev &ev)
 {
     if (!U.timing || !ev.a || !ev.b) return;
-    hipEventRecord(ev.b, U.stream);
+    (void)hipEventRecord(ev.b, U.stream);
     U.evs.push_back(ev);
 }
 
 void harvest_timing()
 {
     for (auto &ev : U.evs) {
-        hipEventSynchronize(ev.b);
+        (void)hipEventSynchronize(ev.b);
         float ms = 0.0f;
-        hipEventElapsedTime(&ms, ev.a, ev.b);
+        (void)hipEventElapsedTime(&ms, ev.a, ev.b);
         if (ev.kind == 0) {
             U.t_force += ms;
             U.t_pairs += ev.pairs;
@@ -218,7 +218,7 @@ int grow_jstore(int need)
         HIPCHK(g6k::launch_copy_store(ns, U.js, old, U.stream), "copy j-store");
         HIPCHK(g6k::launch_reset(ns, old, cap, U.stream), "reset j-store");
         HIPCHK(hipStreamSynchronize(U.stream), "grow sync");
-        hipFree(U.js.base);
+        (void)hipFree(U.js.base);
     } else {
         HIPCHK(g6k::launch_reset(ns, 0, cap, U.stream), "reset j-store");
     }
@@ -237,12 +237,12 @@ int grow_iblock(int need)
     void *base = nullptr;
     hipError_t e = hipMalloc(&base, g6k::iblock_bytes(cap));
     if (e != hipSuccess) return fail(G6X_ERR_NOMEM, "hipMalloc i-block", e);
-    if (U.ib.base) hipFree(U.ib.base);
-    if (U.ib.part_base) hipFree(U.ib.part_base);
+    if (U.ib.base) (void)hipFree(U.ib.base);
+    if (U.ib.part_base) (void)hipFree(U.ib.part_base);
     IBlock nb;
     g6k::iblock_carve(nb, base, cap);
     U.ib = nb;
-    if (U.h_in) { hipHostFree(U.h_in); hipHostFree(U.h_out); }
+    if (U.h_in) { (void)hipHostFree(U.h_in); (void)hipHostFree(U.h_out); }
     U.h_in = nullptr; U.h_out = nullptr;
     e = hipHostMalloc(reinterpret_cast<void **>(&U.h_in), sizeof(double) * 8 * size_t(cap));
     if (e == hipSuccess) e = hipHostMalloc(reinterpret_cast<void **>(&U.h_out), sizeof(double) * 8 * size_t(cap));
@@ -258,7 +258,7 @@ int ensure_partials(int nchunk, int ni)
     const long long need = (long long)nchunk * ni;
     if (nchunk <= 1 || need <= U.ib.part_entries) return 0;
     HIPCHK(hipStreamSynchronize(U.stream), "partials sync");
-    if (U.ib.part_base) hipFree(U.ib.part_base);
+    if (U.ib.part_base) (void)hipFree(U.ib.part_base);
     U.ib.part_base = nullptr;
     U.ib.part = nullptr;
     U.ib.part_entries = 0;
@@ -280,7 +280,7 @@ int flush_stage()
     if (n == 0) return 0;
     if (U.ev_stage) HIPCHK(hipEventSynchronize(U.ev_stage), "stage event");
     if (n > U.h_stage_cap) {
-        if (U.h_stage) hipHostFree(U.h_stage);
+        if (U.h_stage) (void)hipHostFree(U.h_stage);
         size_t cap = std::max<size_t>(n, 4096);
         hipError_t e = hipHostMalloc(reinterpret_cast<void **>(&U.h_stage), sizeof(JRecord) * cap);
         if (e != hipSuccess) { U.h_stage = nullptr; U.h_stage_cap = 0; return fail(G6X_ERR_NOMEM, "hipHostMalloc stage", e); }
@@ -288,7 +288,7 @@ int flush_stage()
     }
     if (n > U.d_stage_cap) {
         HIPCHK(hipStreamSynchronize(U.stream), "stage sync");
-        if (U.d_stage) hipFree(U.d_stage);
+        if (U.d_stage) (void)hipFree(U.d_stage);
         size_t cap = std::max<size_t>(n, 4096);
         hipError_t e = hipMalloc(reinterpret_cast<void **>(&U.d_stage), sizeof(JRecord) * cap);
         if (e != hipSuccess) { U.d_stage = nullptr; U.d_stage_cap = 0; return fail(G6X_ERR_NOMEM, "hipMalloc stage", e); }
@@ -415,13 +415,13 @@ int do_open(int cluster)
     if (dev < 0 || dev >= ndev) return fail(G6X_ERR_NODEVICE, "device id out of range");
     if (U.stream && dev != U.device) {
         // another device: drop the cached allocations of the old one
-        hipSetDevice(U.device);
-        hipStreamSynchronize(U.stream);
-        if (U.js.base) hipFree(U.js.base);
-        if (U.ib.base) hipFree(U.ib.base);
-        if (U.ib.part_base) hipFree(U.ib.part_base);
-        if (U.d_stage) hipFree(U.d_stage);
-        hipStreamDestroy(U.stream);
+        (void)hipSetDevice(U.device);
+        (void)hipStreamSynchronize(U.stream);
+        if (U.js.base) (void)hipFree(U.js.base);
+        if (U.ib.base) (void)hipFree(U.ib.base);
+        if (U.ib.part_base) (void)hipFree(U.ib.part_base);
+        if (U.d_stage) (void)hipFree(U.d_stage);
+        (void)hipStreamDestroy(U.stream);
         U.js = JStore();
         U.ib = IBlock();
         U.d_stage = nullptr;
@@ -590,7 +590,7 @@ G6_API int g6_open_(int *id) { return do_open(id ? *id : 0); }
 
 G6_API int g6_close_(int *)
 {
-    if (U.open && U.stream) hipStreamSynchronize(U.stream);
+    if (U.open && U.stream) (void)hipStreamSynchronize(U.stream);
     U.open = false;
     U.latched = false;
     return 0;
@@ -668,8 +668,8 @@ G6_API int g6_read_neighbour_list_(int *)
     if (ni <= 0) return 0;
     if (ni > U.ngb_cap_i) {
         HIPCHK(hipStreamSynchronize(U.stream), "ngb sync");
-        if (U.d_ngb_counts) hipFree(U.d_ngb_counts);
-        if (U.d_ngb_lists) hipFree(U.d_ngb_lists);
+        if (U.d_ngb_counts) (void)hipFree(U.d_ngb_counts);
+        if (U.d_ngb_lists) (void)hipFree(U.d_ngb_lists);
         U.d_ngb_counts = nullptr;
         U.d_ngb_lists = nullptr;
         U.ngb_cap_i = 0;
@@ -849,7 +849,7 @@ G6_API int g6x_sync(void)
 
 G6_API int g6x_timing(int enable)
 {
-    if (U.stream) hipStreamSynchronize(U.stream);
+    if (U.stream) (void)hipStreamSynchronize(U.stream);
     harvest_timing();
     U.timing = enable != 0;
     U.t_force = U.t_reduce = U.t_pred = U.t_pairs = 0.0;
@@ -860,7 +860,7 @@ G6_API int g6x_timing(int enable)
 G6_API int g6x_get_timing(double *force_ms, double *reduce_ms, double *predict_ms,
                           long long *force_launches, double *pairs)
 {
-    if (U.stream) hipStreamSynchronize(U.stream);
+    if (U.stream) (void)hipStreamSynchronize(U.stream);
     harvest_timing();
     if (force_ms) *force_ms = U.t_force;
     if (reduce_ms) *reduce_ms = U.t_reduce;
@@ -951,7 +951,7 @@ G6_API int g6x_autotune(int nj, int ni_max, double eps2)
                 HIPCHK(hipEventRecord(b, U.stream), "record");
                 HIPCHK(hipEventSynchronize(b), "sync");
                 float ms = 0.0f;
-                hipEventElapsedTime(&ms, a, b);
+                (void)hipEventElapsedTime(&ms, a, b);
                 ms /= reps;
                 timed++;
                 if (ms < best_ms) {
@@ -971,8 +971,8 @@ G6_API int g6x_autotune(int nj, int ni_max, double eps2)
     }
     table.back().ni_hi = kMaxPipes;
 done:
-    hipEventDestroy(a);
-    hipEventDestroy(b);
+    (void)hipEventDestroy(a);
+    (void)hipEventDestroy(b);
     U.tile_njs = save_njs;
     U.tile_ipl = save_ipl;
     U.tile_nchunk = save_nc;

@@ -413,15 +413,15 @@ int read_ctl(Driver *D)
 
 void release(Driver *D)
 {
-    if (D->stream) hipStreamSynchronize(D->stream);
-    if (D->graph) hipGraphExecDestroy(D->graph);
-    if (D->state_base) hipFree(D->state_base);
-    if (D->js.base) hipFree(D->js.base);
-    if (D->ib.base) hipFree(D->ib.base);
-    if (D->part) hipFree(D->part);
-    if (D->ctl) hipFree(D->ctl);
-    if (D->h_ctl) hipHostFree(D->h_ctl);
-    if (D->stream) hipStreamDestroy(D->stream);
+    if (D->stream) (void)hipStreamSynchronize(D->stream);
+    if (D->graph) (void)hipGraphExecDestroy(D->graph);
+    if (D->state_base) (void)hipFree(D->state_base);
+    if (D->js.base) (void)hipFree(D->js.base);
+    if (D->ib.base) (void)hipFree(D->ib.base);
+    if (D->part) (void)hipFree(D->part);
+    if (D->ctl) (void)hipFree(D->ctl);
+    if (D->h_ctl) (void)hipHostFree(D->h_ctl);
+    if (D->stream) (void)hipStreamDestroy(D->stream);
     delete D;
 }
 
@@ -542,18 +542,18 @@ int commit(Driver *D)
 
 int capture(Driver *D, int steps)
 {
-    if (D->graph) hipGraphExecDestroy(D->graph);
+    if (D->graph) (void)hipGraphExecDestroy(D->graph);
     D->graph = nullptr;
     hipGraph_t g;
     DCK(hipStreamBeginCapture(D->stream, hipStreamCaptureModeThreadLocal));
     for (int k = 0; k < steps; k++)
         if (enqueue_step(D, k & 1)) {
-            hipStreamEndCapture(D->stream, &g);
+            (void)hipStreamEndCapture(D->stream, &g);
             return G6X_ERR_HIP;
         }
     DCK(hipStreamEndCapture(D->stream, &g));
     DCK(hipGraphInstantiate(&D->graph, g, nullptr, nullptr, 0));
-    hipGraphDestroy(g);
+    (void)hipGraphDestroy(g);
     D->graph_steps = steps;
     D->graph_prec = D->prec;
     return 0;

@@ -490,7 +490,7 @@ int build_items(Ensemble *E, const int *only = nullptr, bool fill = false)
     }
     const int n = int(order.size());
     if (E->d.item_c) {
-        hipFree(E->d.item_c);
+        (void)hipFree(E->d.item_c);
         E->d.item_c = nullptr;
     }
     CK(hipMalloc(reinterpret_cast<void **>(&E->d.item_c), sizeof(int) * 3 * size_t(std::max(n, 1))));
@@ -505,7 +505,7 @@ int build_items(Ensemble *E, const int *only = nullptr, bool fill = false)
     E->d.nlist = int(plist.size());
     E->items_clusters = covered;
     if (E->graph) {
-        hipGraphExecDestroy(E->graph);
+        (void)hipGraphExecDestroy(E->graph);
         E->graph = nullptr;
     }
     return 0;
@@ -624,11 +624,11 @@ G6C_API void *g6c_create(int nc, const int counts[], const double m[], const dou
     }
     ok &= hipDeviceSynchronize() == hipSuccess;
     int dev = 0, cus = 256;
-    hipGetDevice(&dev);
-    hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
+    (void)hipGetDevice(&dev);
+    (void)hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
     E->persistent_wgs = cus * 4;
     if (!ok || build_items(E)) {
-        hipFree(E->base);
+        (void)hipFree(E->base);
         delete E;
         return nullptr;
     }
@@ -668,18 +668,18 @@ G6C_API int g6c_start(void *h, double t_end)
 
 static int capture_graph(Ensemble *E, int steps_per_graph)
 {
-    if (E->graph) hipGraphExecDestroy(E->graph);
+    if (E->graph) (void)hipGraphExecDestroy(E->graph);
     E->graph = nullptr;
     hipGraph_t g;
     CK(hipStreamBeginCapture(E->stream, hipStreamCaptureModeThreadLocal));
     for (int s = 0; s < steps_per_graph; s++)
         if (enqueue_step(E, E->precision)) {
-            hipStreamEndCapture(E->stream, &g);
+            (void)hipStreamEndCapture(E->stream, &g);
             return G6X_ERR_HIP;
         }
     CK(hipStreamEndCapture(E->stream, &g));
     CK(hipGraphInstantiate(&E->graph, g, nullptr, nullptr, 0));
-    hipGraphDestroy(g);
+    (void)hipGraphDestroy(g);
     E->graph_steps = steps_per_graph;
     E->graph_prec = E->precision;
     return 0;
@@ -697,9 +697,9 @@ G6C_API long long g6c_run(void *h, int steps_per_graph, int use_graph, long long
     std::vector<int> flags(E->d.nc);
     long long rounds = 0;
     hipEvent_t a, b;
-    hipEventCreate(&a);
-    hipEventCreate(&b);
-    hipEventRecord(a, E->stream);
+    CK(hipEventCreate(&a));
+    CK(hipEventCreate(&b));
+    (void)hipEventRecord(a, E->stream);
     long long rc = 0;
     for (;;) {
         if (max_rounds > 0 && rounds >= max_rounds) break;
@@ -732,14 +732,14 @@ G6C_API long long g6c_run(void *h, int steps_per_graph, int use_graph, long long
             }
         }
     }
-    hipEventRecord(b, E->stream);
-    hipEventSynchronize(b);
+    (void)hipEventRecord(b, E->stream);
+    (void)hipEventSynchronize(b);
     float ms = 0;
-    hipEventElapsedTime(&ms, a, b);
+    (void)hipEventElapsedTime(&ms, a, b);
     E->force_ms += ms;
-    hipEventDestroy(a);
-    hipEventDestroy(b);
-    hipHostFree(h_active);
+    (void)hipEventDestroy(a);
+    (void)hipEventDestroy(b);
+    (void)hipHostFree(h_active);
     return rc < 0 ? rc : rounds;
 }
 
@@ -831,8 +831,8 @@ G6C_API int g6c_autotune(void *h)
     std::vector<int> saved(E->d.nc);
     CK(hipMemcpy(saved.data(), E->d.active, sizeof(int) * E->d.nc, hipMemcpyDeviceToHost));
     hipEvent_t a, b;
-    hipEventCreate(&a);
-    hipEventCreate(&b);
+    CK(hipEventCreate(&a));
+    CK(hipEventCreate(&b));
     int timed = 0;
     for (int k = 0; k < kClasses; k++) {
         std::vector<int> act(E->d.nc, 0);
@@ -848,22 +848,22 @@ G6C_API int g6c_autotune(void *h)
             if (build_items(E, act.data())) return G6X_ERR_HIP;
             hipLaunchKernelGGL(kc_predict, dim3(list_blocks(E)), dim3(256), 0, E->stream, E->d, 1);
             launch_force(E, E->precision);
-            hipEventRecord(a, E->stream);
+            (void)hipEventRecord(a, E->stream);
             for (int r = 0; r < 3; r++) {
                 hipLaunchKernelGGL(kc_predict, dim3(list_blocks(E)), dim3(256), 0, E->stream, E->d, 1);
                 launch_force(E, E->precision);
             }
-            hipEventRecord(b, E->stream);
+            (void)hipEventRecord(b, E->stream);
             CK(hipEventSynchronize(b));
             float ms = 0;
-            hipEventElapsedTime(&ms, a, b);
+            (void)hipEventElapsedTime(&ms, a, b);
             timed++;
             if (ms < best) { best = ms; best_njs = njs; }
         }
         E->njs_class[k] = best_njs;
     }
-    hipEventDestroy(a);
-    hipEventDestroy(b);
+    (void)hipEventDestroy(a);
+    (void)hipEventDestroy(b);
     if (build_items(E)) return G6X_ERR_HIP;
     CK(hipMemcpy(E->d.active, saved.data(), sizeof(int) * E->d.nc, hipMemcpyHostToDevice));
     return timed;
@@ -883,10 +883,10 @@ G6C_API void g6c_free(void *h)
 {
     Ensemble *E = static_cast<Ensemble *>(h);
     if (!E) return;
-    hipStreamSynchronize(E->stream);
-    if (E->graph) hipGraphExecDestroy(E->graph);
-    if (E->d.item_c) hipFree(E->d.item_c);
-    if (E->base) hipFree(E->base);
-    hipStreamDestroy(E->stream);
+    (void)hipStreamSynchronize(E->stream);
+    if (E->graph) (void)hipGraphExecDestroy(E->graph);
+    if (E->d.item_c) (void)hipFree(E->d.item_c);
+    if (E->base) (void)hipFree(E->base);
+    (void)hipStreamDestroy(E->stream);
     delete E;
 }
