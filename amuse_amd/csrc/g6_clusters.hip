@@ -165,7 +165,10 @@ __global__ void __launch_bounds__(kWG) kc_force(Dev d)
         const double ix = d.px[0][ip], iy = d.px[1][ip], iz = d.px[2][ip];
         const double ivxd = d.pv[0][ip], ivyd = d.pv[1][ip], ivzd = d.pv[2][ip];
         const float imf = d.fm[ip];
-        float cq = 0.0f;  // max over j of 1 / (collision time)^4
+        // max over j of 1 / (collision time)^4 = max(v^4/s^2, m_ij^2 r^2/s^4),
+        // kept as max(v^2/s) (squared at the end: squaring is monotonic,
+        // rounding included) and max(m_ij^2 r^2/s^4)
+        float cq1 = 0.0f, cq2 = 0.0f;
         for (int tb = 0; tb < n; tb += kTJ) {
             const int jn = min(kTJ, n - tb);
             __syncthreads();
@@ -199,10 +202,9 @@ __global__ void __launch_bounds__(kWG) kc_force(Dev d)
                         // interface.cc:496-516: (s/v^2)^2 and s^4/(m_ij^2 r^2), inverted
                         const float dvx = B.w - ivx, dvy = Cc.x - ivy, dvz = Cc.y - ivz;
                         const float v2 = dvx * dvx + dvy * dvy + dvz * dvz;
-                        const float q1 = (v2 * y2) * (v2 * y2);
+                        cq1 = fmaxf(cq1, v2 * y2);
                         const float u = (imf + A.w) * y2 * y2;
-                        const float q2 = u * u * (s - eps2f);
-                        cq = fmaxf(cq, fmaxf(q1, q2));
+                        cq2 = fmaxf(cq2, u * u * (s - eps2f));
                     }
                     fs[0] = __builtin_fmaf(mi3, dx, fs[0]);
                     fs[1] = __builtin_fmaf(mi3, dy, fs[1]);
@@ -235,7 +237,7 @@ __global__ void __launch_bounds__(kWG) kc_force(Dev d)
                         const double q1 = (v2 * y2) * (v2 * y2);
                         const double u = (double(imf) + A.w) * y2 * y2;
                         const double q2 = u * u * (s - d.eps2);
-                        cq = fmaxf(cq, float(fmax(q1, q2)));
+                        cq2 = fmaxf(cq2, float(fmax(q1, q2)));
                     }
                     ds[0] = __builtin_fma(mi3, dx, ds[0]);
                     ds[1] = __builtin_fma(mi3, dy, ds[1]);
@@ -276,6 +278,7 @@ __global__ void __launch_bounds__(kWG) kc_force(Dev d)
                 }
             }
         }
+        float cq = fmaxf(cq1 * cq1, cq2);
         for (int o = 1; o < njs; o <<= 1) cq = fmaxf(cq, __shfl_xor(cq, o));
         if (ok && js == 0) atomicMax(&d.cmax[c], __float_as_uint(cq));
         if (ok && js == 0) {
