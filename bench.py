@@ -333,6 +333,49 @@ def driver_study(args) -> int:
     return 0
 
 
+def c4_evolve(args) -> int:
+    """Configuration c4 integrated: N = 2^20 Plummer/Salpeter (eps2 = 1e-8),
+    ph4's block steps with the device-resident driver from t = 0 to
+    --c4-t (default 1/64) at --eta (ph4's default 0.14), once with FP64
+    forces and once with mixed ones (initial forces and steps in FP64 for
+    both); energies from FP64 passes.  Reports |dE/E| of each path and their
+    ratio, block and particle steps, wall time and pair interactions/s."""
+    from amuse_amd.blocks import DeviceHermite4
+    n, eps2 = args.n, args.eps2
+    m, pos, vel = make_system(n)
+    res = {"workload": f"c4 evolve: N={n} Plummer/Salpeter, eps2={eps2}, eta={args.eta}, "
+                       f"t=0..{args.c4_t}, device-resident driver"}
+    for prec, name in ((0, "fp64"), (1, "mixed")):
+        d = DeviceHermite4(precision=0)
+        d.commit(m, pos, vel, eps2=eps2, eta=args.eta)
+        e0 = sum(d.energy())
+        d.set_precision(prec)
+        c0 = d.counters()
+        t0 = time.perf_counter()
+        nout = 16
+        for k in range(1, nout + 1):
+            d.evolve(args.c4_t * k / nout)
+            c = d.counters()
+            log(f"  {name}: t={c['system_time']:.6f} blocks={c['block_steps']} "
+                f"{time.perf_counter() - t0:.1f} s")
+        wall = time.perf_counter() - t0
+        c1 = d.counters()
+        d.set_precision(0)
+        e1 = sum(d.energy())
+        d.close()
+        ps = c1["particle_steps"] - c0["particle_steps"]
+        res[name] = {"dE_over_E": (e1 - e0) / abs(e0), "E0": e0, "E1": e1,
+                     "block_steps": c1["block_steps"] - c0["block_steps"],
+                     "particle_steps": ps, "wall_s": wall,
+                     "gpairs_per_s": float(ps) * n / wall / 1e9}
+        log(f"  {name}: {res[name]}")
+    res["energy_error_ratio_mixed_over_fp64"] = abs(res["mixed"]["dE_over_E"]) / max(
+        abs(res["fp64"]["dE_over_E"]), 1e-300)
+    res["speed_ratio_mixed_over_fp64"] = res["fp64"]["wall_s"] / res["mixed"]["wall_s"]
+    print(json.dumps(res))
+    return 0
+
+
 def worker_study(args) -> int:
     """The device-resident integrator out of process through AMUSE's socket
     protocol (amuse_amd/lib/g6_worker): round trip of a scalar call, get_state
@@ -539,11 +582,12 @@ def main() -> int:
     ap.add_argument("--driver-n", default="1024,16384,65536", help="driver: system sizes")
     ap.add_argument("--driver-t", type=float, default=1 / 16, help="driver: end time")
     ap.add_argument("--steps-c3", type=int, default=256)
+    ap.add_argument("--c4-t", type=float, default=1 / 64, help="c4evolve: end time")
     ap.add_argument("--copy-calib", action="store_true",
                     help="only run a plain 1 GiB device copy (HBM counter calibration)")
     ap.add_argument("--workload", default="full",
                     choices=["full", "blocks", "tiles", "ph4", "clusters", "adjoint", "driver",
-                             "worker"],
+                             "worker", "c4evolve"],
                     help="full: full-N force pass at N=2^20 (default); blocks: i-blocks of "
                          "2^k against N=65536 with per-j times, FP64 vs mixed errors")
     args = ap.parse_args()
@@ -563,6 +607,8 @@ def main() -> int:
         return driver_study(args)
     if args.workload == "worker":
         return worker_study(args)
+    if args.workload == "c4evolve":
+        return c4_evolve(args)
     if args.gpus != 1:
         print("bench.py measures one GPU: the force path is built and measured on a single "
               "MI355X (multi-GPU j-domain sharding is a design note only)", file=sys.stderr)
