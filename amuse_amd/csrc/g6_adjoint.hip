@@ -24,6 +24,7 @@
 // tile.  No allocation: the caller passes every buffer and its stream.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <type_traits>
 
 #define G6A_API extern "C" __attribute__((visibility("default")))
@@ -294,7 +295,157 @@ int launch_adjoint(int n, const T *x, const T *v, const T *m, const T *ga, const
     return hipGetLastError() == hipSuccess ? 0 : -5;
 }
 
+// ---------------------------------------------------------------------------
+// The fixed-step Hermite scheme around the pair sum (src/amuse_hermite/
+// interface.cc: predictor :266-281, time-symmetric corrector :536-556) as
+// elementwise kernels over the 3n components, and the transposes of both
+// maps for the backward pass (all linear in their inputs):
+//   xp = x + v dt + a dt^2/2 + j dt^3/6,   vp = v + a dt + j dt^2/2
+//   v1 = v + (a + a1) dt/2 + (j - j1) dt^2/12
+//   x1 = x + (v + v1) dt/2 + (a - a1) dt^2/12
+// ---------------------------------------------------------------------------
+template <class T>
+__global__ void __launch_bounds__(256) k_step_predict(int m3, const T *__restrict__ x,
+                                                      const T *__restrict__ v,
+                                                      const T *__restrict__ a,
+                                                      const T *__restrict__ j, T dt, T c2, T c3,
+                                                      T *__restrict__ xp, T *__restrict__ vp)
+{
+    for (int k = blockIdx.x * blockDim.x + threadIdx.x; k < m3; k += gridDim.x * blockDim.x) {
+        xp[k] = x[k] + v[k] * dt + a[k] * c2 + j[k] * c3;
+        vp[k] = v[k] + a[k] * dt + j[k] * c2;
+    }
+}
+
+template <class T>
+__global__ void __launch_bounds__(256) k_step_correct(int m3, const T *__restrict__ x,
+                                                      const T *__restrict__ v,
+                                                      const T *__restrict__ a,
+                                                      const T *__restrict__ j,
+                                                      const T *__restrict__ a1,
+                                                      const T *__restrict__ j1, T h, T q,
+                                                      T *__restrict__ x1, T *__restrict__ v1)
+{
+    for (int k = blockIdx.x * blockDim.x + threadIdx.x; k < m3; k += gridDim.x * blockDim.x) {
+        const T vn = v[k] + (a[k] + a1[k]) * h + (j[k] - j1[k]) * q;
+        x1[k] = x[k] + (v[k] + vn) * h + (a[k] - a1[k]) * q;
+        v1[k] = vn;
+    }
+}
+
+// Transpose of the corrector (h = dt/2, q = dt^2/12): from dL/dx1, dL/dv1
+// and the outer dL/da1, dL/dj1 (NULL: zero) to dL/dx, dv, da, dj and the
+// totals dL/da1, dL/dj1 that enter the pair-sum adjoint.
+template <class T>
+__global__ void __launch_bounds__(256) k_step_correct_t(int m3, const T *__restrict__ gx1,
+                                                        const T *__restrict__ gv1,
+                                                        const T *__restrict__ ga1e,
+                                                        const T *__restrict__ gj1e, T h, T q,
+                                                        T *__restrict__ gx, T *__restrict__ gv,
+                                                        T *__restrict__ ga, T *__restrict__ gj,
+                                                        T *__restrict__ ga1, T *__restrict__ gj1)
+{
+    for (int k = blockIdx.x * blockDim.x + threadIdx.x; k < m3; k += gridDim.x * blockDim.x) {
+        const T bx = gx1[k];
+        const T bv = gv1[k] + bx * h;  // dL/dv1 with x1's dependence on v1
+        gx[k] = bx;
+        gv[k] = bx * h + bv;
+        ga[k] = bx * q + bv * h;
+        gj[k] = bv * q;
+        ga1[k] = (ga1e ? ga1e[k] : T(0)) - bx * q + bv * h;
+        gj1[k] = (gj1e ? gj1e[k] : T(0)) - bv * q;
+    }
+}
+
+// Transpose of the predictor, added into gx, gv, ga, gj.
+template <class T>
+__global__ void __launch_bounds__(256) k_step_predict_t(int m3, const T *__restrict__ gxp,
+                                                        const T *__restrict__ gvp, T dt, T c2,
+                                                        T c3, T *__restrict__ gx,
+                                                        T *__restrict__ gv, T *__restrict__ ga,
+                                                        T *__restrict__ gj)
+{
+    for (int k = blockIdx.x * blockDim.x + threadIdx.x; k < m3; k += gridDim.x * blockDim.x) {
+        const T bx = gxp[k], bv = gvp[k];
+        gx[k] += bx;
+        gv[k] += bx * dt + bv;
+        ga[k] += bx * c2 + bv * dt;
+        gj[k] += bx * c3 + bv * c2;
+    }
+}
+
+inline int elem_grid(int m3) { return std::min((m3 + 255) / 256, 2048); }
+
+template <class T>
+int step_predict(int n, const T *x, const T *v, const T *a, const T *j, double dt, T *xp, T *vp,
+                 hipStream_t st)
+{
+    if (n <= 0) return 0;
+    hipLaunchKernelGGL(k_step_predict<T>, dim3(elem_grid(3 * n)), dim3(256), 0, st, 3 * n, x, v,
+                       a, j, T(dt), T(dt * dt / 2), T(dt * dt * dt / 6), xp, vp);
+    return hipGetLastError() == hipSuccess ? 0 : -5;
+}
+
+template <class T>
+int step_correct(int n, const T *x, const T *v, const T *a, const T *j, const T *a1,
+                 const T *j1, double dt, T *x1, T *v1, hipStream_t st)
+{
+    if (n <= 0) return 0;
+    hipLaunchKernelGGL(k_step_correct<T>, dim3(elem_grid(3 * n)), dim3(256), 0, st, 3 * n, x, v,
+                       a, j, a1, j1, T(dt / 2), T(dt * dt / 12), x1, v1);
+    return hipGetLastError() == hipSuccess ? 0 : -5;
+}
+
+template <class T>
+int step_correct_t(int n, const T *gx1, const T *gv1, const T *ga1e, const T *gj1e, double dt,
+                   T *gx, T *gv, T *ga, T *gj, T *ga1, T *gj1, hipStream_t st)
+{
+    if (n <= 0) return 0;
+    hipLaunchKernelGGL(k_step_correct_t<T>, dim3(elem_grid(3 * n)), dim3(256), 0, st, 3 * n, gx1,
+                       gv1, ga1e, gj1e, T(dt / 2), T(dt * dt / 12), gx, gv, ga, gj, ga1, gj1);
+    return hipGetLastError() == hipSuccess ? 0 : -5;
+}
+
+template <class T>
+int step_predict_t(int n, const T *gxp, const T *gvp, double dt, T *gx, T *gv, T *ga, T *gj,
+                   hipStream_t st)
+{
+    if (n <= 0) return 0;
+    hipLaunchKernelGGL(k_step_predict_t<T>, dim3(elem_grid(3 * n)), dim3(256), 0, st, 3 * n, gxp,
+                       gvp, T(dt), T(dt * dt / 2), T(dt * dt * dt / 6), gx, gv, ga, gj);
+    return hipGetLastError() == hipSuccess ? 0 : -5;
+}
+
 }  // namespace g6a
+
+#define G6A_STEP_API(S, T)                                                                      \
+    G6A_API int g6a_step_predict_##S(int n, const T *x, const T *v, const T *a, const T *j,      \
+                                     double dt, T *xp, T *vp, void *stream)                     \
+    {                                                                                          \
+        return g6a::step_predict<T>(n, x, v, a, j, dt, xp, vp, static_cast<hipStream_t>(stream)); \
+    }                                                                                          \
+    G6A_API int g6a_step_correct_##S(int n, const T *x, const T *v, const T *a, const T *j,      \
+                                     const T *a1, const T *j1, double dt, T *x1, T *v1,         \
+                                     void *stream)                                              \
+    {                                                                                          \
+        return g6a::step_correct<T>(n, x, v, a, j, a1, j1, dt, x1, v1,                          \
+                                    static_cast<hipStream_t>(stream));                          \
+    }                                                                                          \
+    G6A_API int g6a_step_correct_t_##S(int n, const T *gx1, const T *gv1, const T *ga1e,         \
+                                       const T *gj1e, double dt, T *gx, T *gv, T *ga, T *gj,    \
+                                       T *ga1, T *gj1, void *stream)                            \
+    {                                                                                          \
+        return g6a::step_correct_t<T>(n, gx1, gv1, ga1e, gj1e, dt, gx, gv, ga, gj, ga1, gj1,    \
+                                      static_cast<hipStream_t>(stream));                        \
+    }                                                                                          \
+    G6A_API int g6a_step_predict_t_##S(int n, const T *gxp, const T *gvp, double dt, T *gx,     \
+                                       T *gv, T *ga, T *gj, void *stream)                       \
+    {                                                                                          \
+        return g6a::step_predict_t<T>(n, gxp, gvp, dt, gx, gv, ga, gj,                          \
+                                      static_cast<hipStream_t>(stream));                        \
+    }
+G6A_STEP_API(f32, float)
+G6A_STEP_API(f64, double)
 
 G6A_API int g6a_forces_f32(int n, const float *x, const float *v, const float *m, float eps2,
                            float *acc, float *jerk, void *stream)

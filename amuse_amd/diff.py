@@ -5,11 +5,16 @@
   lib/g6lib/g6lib.c:165-183) and g6a_adjoint_* (their vector-Jacobian
   product: dL/dx, dL/dv, dL/dm from dL/dacc, dL/djerk).  float32 and float64
   tensors on the GPU; no CPU fallback.
-* :func:`hermite_integrate` — fixed-step 4th-order Hermite, the scheme of
+* :class:`HermiteStep` — one fixed 4th-order Hermite step, the scheme of
   src/amuse_hermite/interface.cc (predictor :266-281, time-symmetric
-  corrector :536-556, fixed step as in its test mode :618-637), with the
-  state checkpointed every `checkpoint` steps and the steps in between
-  recomputed in the backward pass.
+  corrector :536-556, fixed step as in its test mode :618-637): predictor,
+  pair sum and corrector as three kernels (g6a_step_predict_*, g6a_forces_*,
+  g6a_step_correct_*), and in the backward pass the transposes of the two
+  linear maps around the pair-sum adjoint (g6a_step_correct_t_*,
+  g6a_adjoint_*, g6a_step_predict_t_*).
+* :func:`hermite_integrate` — HermiteStep repeated, with the state
+  checkpointed every `checkpoint` steps and the steps in between recomputed
+  in the backward pass.
 * :func:`lagrangian_radii` and :func:`radii_loss` — mass radii about the
   centre of mass (the quantity of src/amuse/datamodel/particle_attributes.py
   LagrangianRadii, here about the COM), linearly interpolated in the
@@ -32,6 +37,7 @@ def _kernels():
     if _lib is None:
         L = load_library()
         vp = C.c_void_p
+        d = C.c_double
         for name, real in (("f32", C.c_float), ("f64", C.c_double)):
             f = getattr(L, "g6a_forces_" + name)
             f.restype = C.c_int
@@ -39,6 +45,15 @@ def _kernels():
             g = getattr(L, "g6a_adjoint_" + name)
             g.restype = C.c_int
             g.argtypes = [C.c_int, vp, vp, vp, vp, vp, real, vp, vp, vp, vp]
+            for fn, args in (("g6a_step_predict_", [C.c_int, vp, vp, vp, vp, d, vp, vp, vp]),
+                             ("g6a_step_correct_", [C.c_int, vp, vp, vp, vp, vp, vp, d, vp, vp,
+                                                    vp]),
+                             ("g6a_step_correct_t_", [C.c_int, vp, vp, vp, vp, d, vp, vp, vp, vp,
+                                                      vp, vp, vp]),
+                             ("g6a_step_predict_t_", [C.c_int, vp, vp, d, vp, vp, vp, vp, vp])):
+                h = getattr(L, fn + name)
+                h.restype = C.c_int
+                h.argtypes = args
         _lib = L
     return _lib
 
@@ -100,8 +115,73 @@ def pair_force(x, v, m, eps2):
     return PairForce.apply(x, v, m, eps2)
 
 
+def _ptr(t):
+    return None if t is None else t.data_ptr()
+
+
+def _call(name, dtype, *args):
+    rc = getattr(_kernels(), name + _suffix(dtype))(*args)
+    if rc != 0:
+        raise RuntimeError(f"{name}{_suffix(dtype)} failed ({rc})")
+
+
+class HermiteStep(torch.autograd.Function):
+    """x1, v1, a1, j1 = HermiteStep.apply(x, v, a, j, m, dt, eps2): one fixed
+    step (src/amuse_hermite/interface.cc:266-281, 536-556) in three kernels;
+    the backward pass runs the transposes of the predictor and corrector
+    around the pair-sum adjoint."""
+
+    @staticmethod
+    def forward(ctx, x, v, a, j, m, dt: float, eps2: float):
+        x, v, a, j, m = (_check(t, n) for t, n in ((x, "x"), (v, "v"), (a, "a"), (j, "j"),
+                                                    (m, "m")))
+        n = x.shape[0]
+        stream = torch.cuda.current_stream(x.device).cuda_stream
+        xp, vp = torch.empty_like(x), torch.empty_like(v)
+        _call("g6a_step_predict_", x.dtype, n, x.data_ptr(), v.data_ptr(), a.data_ptr(),
+              j.data_ptr(), float(dt), xp.data_ptr(), vp.data_ptr(), stream)
+        a1, j1 = torch.empty_like(x), torch.empty_like(x)
+        _call("g6a_forces_", x.dtype, n, xp.data_ptr(), vp.data_ptr(), m.data_ptr(), float(eps2),
+              a1.data_ptr(), j1.data_ptr(), stream)
+        x1, v1 = torch.empty_like(x), torch.empty_like(v)
+        _call("g6a_step_correct_", x.dtype, n, x.data_ptr(), v.data_ptr(), a.data_ptr(),
+              j.data_ptr(), a1.data_ptr(), j1.data_ptr(), float(dt), x1.data_ptr(),
+              v1.data_ptr(), stream)
+        ctx.save_for_backward(xp, vp, m)
+        ctx.dt, ctx.eps2 = float(dt), float(eps2)
+        return x1, v1, a1, j1
+
+    @staticmethod
+    def backward(ctx, gx1, gv1, ga1, gj1):
+        xp, vp, m = ctx.saved_tensors
+        n = xp.shape[0]
+        stream = torch.cuda.current_stream(xp.device).cuda_stream
+        gx1 = torch.zeros_like(xp) if gx1 is None else gx1.contiguous()
+        gv1 = torch.zeros_like(xp) if gv1 is None else gv1.contiguous()
+        ga1 = None if ga1 is None else ga1.contiguous()
+        gj1 = None if gj1 is None else gj1.contiguous()
+        gx, gv, ga, gj = (torch.empty_like(xp) for _ in range(4))
+        ga1t, gj1t = torch.empty_like(xp), torch.empty_like(xp)
+        _call("g6a_step_correct_t_", xp.dtype, n, gx1.data_ptr(), gv1.data_ptr(), _ptr(ga1),
+              _ptr(gj1), ctx.dt, gx.data_ptr(), gv.data_ptr(), ga.data_ptr(), gj.data_ptr(),
+              ga1t.data_ptr(), gj1t.data_ptr(), stream)
+        gxp, gvp, gm = torch.empty_like(xp), torch.empty_like(vp), torch.empty_like(m)
+        _call("g6a_adjoint_", xp.dtype, n, xp.data_ptr(), vp.data_ptr(), m.data_ptr(),
+              ga1t.data_ptr(), gj1t.data_ptr(), ctx.eps2, gxp.data_ptr(), gvp.data_ptr(),
+              gm.data_ptr(), stream)
+        _call("g6a_step_predict_t_", xp.dtype, n, gxp.data_ptr(), gvp.data_ptr(), ctx.dt,
+              gx.data_ptr(), gv.data_ptr(), ga.data_ptr(), gj.data_ptr(), stream)
+        return gx, gv, ga, gj, gm, None, None
+
+
 def hermite_step(x, v, a, j, m, dt: float, eps2: float):
-    """One fixed step (src/amuse_hermite/interface.cc:266-281, 536-556)."""
+    """One fixed step through the fused kernels (HermiteStep)."""
+    return HermiteStep.apply(x, v, a, j, m, dt, eps2)
+
+
+def hermite_step_reference(x, v, a, j, m, dt: float, eps2: float):
+    """The same step as torch operations around PairForce (the reference the
+    fused step is tested against)."""
     xp = x + v * dt + a * (dt * dt / 2) + j * (dt * dt * dt / 6)
     vp = v + a * dt + j * (dt * dt / 2)
     a1, j1 = pair_force(xp, vp, m, eps2)

@@ -514,7 +514,7 @@ def adjoint_c3(args) -> int:
         torch.cuda.synchronize()
         t2 = time.perf_counter()
         g = (xx.grad, vv.grad, mm.grad) if grads else None
-        return float(loss), g, t1 - t0, t2 - t1
+        return float(loss.detach()), g, t1 - t0, t2 - t1
 
     run(torch.float32)   # warm-up
     L32, g32, tf32, tb32 = run(torch.float32)

@@ -139,6 +139,34 @@ int g6a_adjoint_f64(int n, const double *x, const double *v, const double *m,
                     const double *ga, const double *gj, double eps2,
                     double *gx, double *gv, double *gm, void *stream);
 
+/* Fixed-step Hermite scheme around the pair sum (src/amuse_hermite/
+ * interface.cc:266-281 predictor, :536-556 corrector) on the same [n][3]
+ * device arrays, and the transposes of both linear maps for the backward
+ * pass: g6a_step_correct_t_* maps dL/dx1, dL/dv1 and the outer dL/da1,
+ * dL/dj1 (NULL: zero) to dL/dx, dv, da, dj and the total dL/da1, dL/dj1 that
+ * enter g6a_adjoint_*; g6a_step_predict_t_* adds the predictor's part for
+ * dL/dxp, dL/dvp into dL/dx, dv, da, dj.  Nothing is allocated. */
+int g6a_step_predict_f32(int n, const float *x, const float *v, const float *a,
+                         const float *j, double dt, float *xp, float *vp, void *stream);
+int g6a_step_predict_f64(int n, const double *x, const double *v, const double *a,
+                         const double *j, double dt, double *xp, double *vp, void *stream);
+int g6a_step_correct_f32(int n, const float *x, const float *v, const float *a,
+                         const float *j, const float *a1, const float *j1, double dt,
+                         float *x1, float *v1, void *stream);
+int g6a_step_correct_f64(int n, const double *x, const double *v, const double *a,
+                         const double *j, const double *a1, const double *j1, double dt,
+                         double *x1, double *v1, void *stream);
+int g6a_step_correct_t_f32(int n, const float *gx1, const float *gv1, const float *ga1e,
+                           const float *gj1e, double dt, float *gx, float *gv, float *ga,
+                           float *gj, float *ga1, float *gj1, void *stream);
+int g6a_step_correct_t_f64(int n, const double *gx1, const double *gv1, const double *ga1e,
+                           const double *gj1e, double dt, double *gx, double *gv,
+                           double *ga, double *gj, double *ga1, double *gj1, void *stream);
+int g6a_step_predict_t_f32(int n, const float *gxp, const float *gvp, double dt, float *gx,
+                           float *gv, float *ga, float *gj, void *stream);
+int g6a_step_predict_t_f64(int n, const double *gxp, const double *gvp, double dt,
+                           double *gx, double *gv, double *ga, double *gj, void *stream);
+
 /* ---- Device-resident block-step Hermite integrator (g6_blocks.hip): the
  * integration loop of ph4's GPU path (scheduler, i-predictor, force call,
  * Hermite corrector with the Aarseth step; src/amuse_ph4/src/jdata.cc:
