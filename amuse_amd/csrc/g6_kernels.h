@@ -113,7 +113,17 @@ struct StepPlan {
     int jchunk;   // j per chunk (multiple of kTJ)
 };
 
-constexpr long long kStepPartBudget = 1ll << 21;  // nchunk * ni bound on partials
+// work items a large step aims at (build-time tunable) and the nchunk * ni
+// bound on its partials (4096: +2 % on N = 2^20 block steps against 2048,
+// the same at N = 16384 and 65536; 8192 no better)
+#ifndef G6_STEP_ITEMS
+#define G6_STEP_ITEMS 4096
+#endif
+#ifndef G6_STEP_PART_LOG2
+#define G6_STEP_PART_LOG2 22
+#endif
+constexpr int kStepItems = G6_STEP_ITEMS;
+constexpr long long kStepPartBudget = 1ll << G6_STEP_PART_LOG2;
 constexpr int kStepGrid = 2048;                   // persistent workgroups of a step
 
 __host__ __device__ inline int step_ib(int shape)
@@ -129,7 +139,7 @@ __host__ __device__ inline int step_njs(int shape)
 // 64 lanes per i and no j-split over workgroups, so a step is one short
 // force kernel and no reduce.  Otherwise the rule of the ABI path's
 // automatic choice: the widest i-block not wider than the next power of two
-// of ni, then j-chunks for ~8 workgroups per CU within the partials budget
+// of ni, then j-chunks for ~kStepItems work items within the partials budget
 // (measured better than 64 lanes per i from nj = 16384 on: with 4 i per
 // workgroup the j-tiles are loaded once per 4 i and the loads dominate).
 __host__ __device__ inline StepPlan plan_step(int ni, int nj)
@@ -150,7 +160,7 @@ __host__ __device__ inline StepPlan plan_step(int ni, int nj)
     const int ib = step_ib(sp.shape);
     sp.iblocks = (ni + ib - 1) / ib;
     const int max_chunks = (nj + kTJ - 1) / kTJ;
-    int nchunk = sp.iblocks > 0 ? (2048 + sp.iblocks - 1) / sp.iblocks : 1;
+    int nchunk = sp.iblocks > 0 ? (kStepItems + sp.iblocks - 1) / sp.iblocks : 1;
     if (nchunk > max_chunks) nchunk = max_chunks;
     while (nchunk > 1 && (long long)nchunk * ni > kStepPartBudget) nchunk /= 2;
     int jchunk = (nj + nchunk - 1) / nchunk;
