@@ -37,6 +37,11 @@ using g6k::JStore;
 constexpr int kMaxPipes = 1 << 22;
 constexpr int kMaxSlots = 1 << 27;
 constexpr long long kPartBudget = 1ll << 24;  // nchunk * ni limit for partials (<= 2 GB)
+// i-blocks and j-stages up to this many rows go between host and kernels
+// through pinned host memory that the kernels read and write themselves (no
+// copy launches: at the block sizes of a block-step integrator the copies'
+// latency, not their bytes, is the cost); larger ones are copied.
+constexpr int kDirectRows = 4096;
 
 struct TuneEntry {
     int ni_lo, ni_hi;  // bucket [lo, hi]
@@ -56,6 +61,7 @@ struct Unit {
     std::vector<JRecord> stage;
     std::vector<int> stage_pos;  // slot -> index in stage, -1 if none
     JRecord *h_stage = nullptr;
+    JRecord *d_hstage = nullptr;  // the device's address of h_stage
     size_t h_stage_cap = 0;
     JRecord *d_stage = nullptr;
     size_t d_stage_cap = 0;
@@ -63,11 +69,17 @@ struct Unit {
 
     // pinned mirrors of the device i-block rows (same row pitch): [8][cap]
     // x y z vx vy vz h2 id and [8][cap] acc3 jerk3 pot nn; id and nn are
-    // ints in the first half of their row
+    // ints in the first half of their row.  Blocks of at most kDirectRows
+    // are read and written there by the kernels themselves (d_in, d_out:
+    // the device's addresses of the same memory); larger ones are copied.
     double *h_in = nullptr;
     double *h_out = nullptr;
+    double *d_in = nullptr;
+    double *d_out = nullptr;
     int h_cap = 0;
-    hipEvent_t ev_in = nullptr;
+    hipEvent_t ev_in = nullptr;  // after the last use of h_in by the device
+    bool in_direct = false;      // the latched i-block is read from h_in
+    bool out_direct = false;     // the last launch wrote its results into h_out
 
     double ti = 0.0;
     bool pred_valid = false;
@@ -247,6 +259,8 @@ int grow_iblock(int need)
     e = hipHostMalloc(reinterpret_cast<void **>(&U.h_in), sizeof(double) * 8 * size_t(cap));
     if (e == hipSuccess) e = hipHostMalloc(reinterpret_cast<void **>(&U.h_out), sizeof(double) * 8 * size_t(cap));
     if (e != hipSuccess) return fail(G6X_ERR_NOMEM, "hipHostMalloc i/o", e);
+    HIPCHK(hipHostGetDevicePointer(reinterpret_cast<void **>(&U.d_in), U.h_in, 0), "i/o map");
+    HIPCHK(hipHostGetDevicePointer(reinterpret_cast<void **>(&U.d_out), U.h_out, 0), "i/o map");
     U.h_cap = cap;
     return 0;
 }
@@ -285,8 +299,9 @@ int flush_stage()
         hipError_t e = hipHostMalloc(reinterpret_cast<void **>(&U.h_stage), sizeof(JRecord) * cap);
         if (e != hipSuccess) { U.h_stage = nullptr; U.h_stage_cap = 0; return fail(G6X_ERR_NOMEM, "hipHostMalloc stage", e); }
         U.h_stage_cap = cap;
+        HIPCHK(hipHostGetDevicePointer(reinterpret_cast<void **>(&U.d_hstage), U.h_stage, 0), "stage map");
     }
-    if (n > U.d_stage_cap) {
+    if (n > size_t(kDirectRows) && n > U.d_stage_cap) {
         HIPCHK(hipStreamSynchronize(U.stream), "stage sync");
         if (U.d_stage) (void)hipFree(U.d_stage);
         size_t cap = std::max<size_t>(n, 4096);
@@ -295,9 +310,15 @@ int flush_stage()
         U.d_stage_cap = cap;
     }
     memcpy(U.h_stage, U.stage.data(), sizeof(JRecord) * n);
-    HIPCHK(hipMemcpyAsync(U.d_stage, U.h_stage, sizeof(JRecord) * n, hipMemcpyHostToDevice, U.stream), "stage H2D");
+    // small stages are read by the scatter kernel where they lie (pinned
+    // host memory); large ones are copied to the device first
+    const JRecord *rec = U.d_hstage;
+    if (n > size_t(kDirectRows)) {
+        HIPCHK(hipMemcpyAsync(U.d_stage, U.h_stage, sizeof(JRecord) * n, hipMemcpyHostToDevice, U.stream), "stage H2D");
+        rec = U.d_stage;
+    }
     const bool pv = U.pred_valid && U.pred_prec == U.prec && U.pred_compat == U.compat;
-    HIPCHK(g6k::launch_scatter(U.js, U.d_stage, int(n), U.prec, pv ? 1 : 0, U.pred_ti, U.compat, U.stream), "scatter");
+    HIPCHK(g6k::launch_scatter(U.js, rec, int(n), U.prec, pv ? 1 : 0, U.pred_ti, U.compat, U.stream), "scatter");
     if (!U.ev_stage) HIPCHK(hipEventCreateWithFlags(&U.ev_stage, hipEventDisableTiming), "event");
     HIPCHK(hipEventRecord(U.ev_stage, U.stream), "stage record");
     for (const auto &r : U.stage) U.stage_pos[r.addr] = -1;
@@ -335,6 +356,13 @@ ForceCfg choose_cfg(int ni, int nj)
                 njs = t.njs; ipl = t.ipl; nchunk = t.nchunk; smem = t.smem;
                 break;
             }
+    }
+    if (njs == 0 && nj <= 8 * g6k::kTJ) {
+        // Short j-ranges: one i per wave over all j, no j-split and so no
+        // reduce kernel (the small-system plan of the block-step driver);
+        // a split j-range gives a few workgroups a tile each and costs a
+        // reduce launch that the few tiles saved do not pay for.
+        njs = 64; ipl = 1; nchunk = 1; smem = 0;
     }
     if (njs == 0) {
         // Largest i-block that does not leave lanes idle, then enough
@@ -377,22 +405,57 @@ ForceCfg choose_cfg(int ni, int nj)
     return c;
 }
 
+// Records ev_in: h_in may be rewritten once the work queued so far is done.
+hipError_t record_in()
+{
+    if (!U.ev_in) {
+        hipError_t e = hipEventCreateWithFlags(&U.ev_in, hipEventDisableTiming);
+        if (e != hipSuccess) return e;
+    }
+    return hipEventRecord(U.ev_in, U.stream);
+}
+
+// The i-block as the kernels see it: the device rows, or for a latched
+// block read in place the pinned input rows; the result rows in pinned
+// memory for blocks of at most kDirectRows (same row pitch either way).
+IBlock io_view(int ni)
+{
+    IBlock b = U.ib;
+    const size_t C = size_t(U.ib.cap);
+    if (U.in_direct) {
+        for (int k = 0; k < 3; k++) {
+            b.x[k] = U.d_in + k * C;
+            b.v[k] = U.d_in + (3 + k) * C;
+        }
+        b.h2 = U.d_in + 6 * C;
+        b.id = reinterpret_cast<int *>(U.d_in + 7 * C);
+    }
+    if (ni <= kDirectRows) {
+        b.out = U.d_out;
+        b.nn = reinterpret_cast<int *>(U.d_out + 7 * C);
+    }
+    return b;
+}
+
 int run_force(int ni, int nj, double eps2, bool want_nn)
 {
     if (ni <= 0) { U.last_ni = 0; return 0; }
     ForceCfg cfg = choose_cfg(ni, std::max(nj, 1));
     int rc = ensure_partials(cfg.nchunk, ni);
     if (rc) return rc;
+    const IBlock b = io_view(ni);
     Unit::Ev ev;
     timed_begin(ev, 0, double(ni) * double(nj));
-    HIPCHK(g6k::launch_force(U.js, U.ib, ni, nj, eps2, U.prec, want_nn, cfg, U.stream), "force kernel");
+    HIPCHK(g6k::launch_force(U.js, b, ni, nj, eps2, U.prec, want_nn, cfg, U.stream), "force kernel");
     timed_end(ev);
     if (cfg.nchunk > 1) {
         Unit::Ev ev2;
         timed_begin(ev2, 1, 0.0);
-        HIPCHK(g6k::launch_reduce(U.js, U.ib, ni, cfg.nchunk, U.prec, want_nn, U.stream), "reduce kernel");
+        HIPCHK(g6k::launch_reduce(U.js, b, ni, cfg.nchunk, U.prec, want_nn, U.stream), "reduce kernel");
         timed_end(ev2);
     }
+    U.out_direct = b.out != U.ib.out;
+    if (U.in_direct) HIPCHK(record_in(), "i record");
     U.launched = true;
     U.launched_nn = want_nn;
     U.last_ni = ni;
@@ -522,12 +585,18 @@ int do_firsthalf(int nj, int ni, const int *index, const double (*xi)[3], const 
             U.h_in[6 * C + i] = h2 ? h2[i] : 0.0;
             h_id[i] = index[i];
         }
-        // all eight rows in one pitched copy (the id row's second half is
-        // copied along, unused)
-        HIPCHK(hipMemcpy2DAsync(U.ib.x[0], sizeof(double) * C, U.h_in, sizeof(double) * C,
-                                sizeof(double) * ni, 8, hipMemcpyHostToDevice, U.stream), "i H2D");
-        if (!U.ev_in) HIPCHK(hipEventCreateWithFlags(&U.ev_in, hipEventDisableTiming), "event");
-        HIPCHK(hipEventRecord(U.ev_in, U.stream), "i record");
+        // read in place when the force launch reads each row a few times at
+        // most (once per j-chunk of its i-block), else copied once
+        const ForceCfg cfg = choose_cfg(ni, std::max(nj, 1));
+        U.in_direct = (long long)cfg.nchunk * ni <= kDirectRows;
+        if (!U.in_direct) {
+            // all eight rows in one pitched copy (the id row's second half
+            // is copied along, unused)
+            HIPCHK(hipMemcpy2DAsync(U.ib.x[0], sizeof(double) * C, U.h_in, sizeof(double) * C,
+                                    sizeof(double) * ni, 8, hipMemcpyHostToDevice, U.stream),
+                   "i H2D");
+            HIPCHK(record_in(), "i record");
+        }
     }
     U.l_ni = ni;
     U.l_nj = nj;
@@ -545,10 +614,13 @@ int fetch_results(int ni, double (*acc)[3], double (*jerk)[3], double *pot, int 
 {
     if (ni <= 0) return 0;
     const size_t C = size_t(U.ib.cap);
-    // result rows acc3 jerk3 pot (nn) follow each other at pitch C: one copy
-    HIPCHK(hipMemcpy2DAsync(U.h_out, sizeof(double) * C, U.ib.out, sizeof(double) * C,
-                            sizeof(double) * ni, want_nn ? 8 : 7, hipMemcpyDeviceToHost, U.stream),
-           "out D2H");
+    // result rows acc3 jerk3 pot (nn) follow each other at pitch C: written
+    // there by the kernels, or one copy
+    if (!U.out_direct)
+        HIPCHK(hipMemcpy2DAsync(U.h_out, sizeof(double) * C, U.ib.out, sizeof(double) * C,
+                                sizeof(double) * ni, want_nn ? 8 : 7, hipMemcpyDeviceToHost,
+                                U.stream),
+               "out D2H");
     const int *h_nn = reinterpret_cast<const int *>(U.h_out + 7 * C);
     HIPCHK(hipStreamSynchronize(U.stream), "lasthalf sync");
     for (int i = 0; i < ni; i++) {
@@ -681,6 +753,14 @@ G6_API int g6_read_neighbour_list_(int *)
         U.ngb_cap_i = cap;
     }
     HIPCHK(hipMemsetAsync(U.d_ngb_counts, 0, sizeof(int) * size_t(ni), U.stream), "ngb clear");
+    if (U.in_direct) {
+        // the query reads each i once per j-tile: the rows go to the device
+        const size_t C = size_t(U.ib.cap);
+        HIPCHK(hipMemcpy2DAsync(U.ib.x[0], sizeof(double) * C, U.h_in, sizeof(double) * C,
+                                sizeof(double) * ni, 8, hipMemcpyHostToDevice, U.stream), "i H2D");
+        HIPCHK(record_in(), "i record");
+        U.in_direct = false;
+    }
     HIPCHK(g6k::launch_ngb(U.js, U.ib, ni, U.l_nj, U.prec, U.d_ngb_counts, U.d_ngb_lists, U.stream), "ngb kernel");
     U.h_ngb_counts.resize(ni);
     U.h_ngb_lists.resize(size_t(ni) * g6k::kNgbCap);
@@ -828,6 +908,7 @@ G6_API int g6x_calc_self(int nj, int i0, int ni, double eps2, int want_nn)
     if (rc) return rc;
     HIPCHK(g6k::launch_iblock_self(U.js, U.ib, i0, ni, U.prec, U.stream), "iblock self");
     U.latched = false;
+    U.in_direct = false;  // the i-block is in the device rows
     return run_force(ni, nj, eps2, want_nn != 0);
 }
 
