@@ -344,6 +344,22 @@ int ensure_predicted()
 // ---------------------------------------------------------------------------
 // Tile choice
 // ---------------------------------------------------------------------------
+// j-chunks of the automatic choice for `iblocks` i-blocks of an ni x nj
+// launch: ~8 workgroups per CU at least; for long j-ranges up to ~128, so a
+// launch is many rounds of workgroups and the last round's imbalance is
+// small (at ni = nj = 2^20, two rounds lost 5 % against 32), with every
+// chunk still >= 16 tiles; at most one chunk per tile, and within the
+// partials budget.
+int auto_chunks(int iblocks, int ni, int nj)
+{
+    const int max_chunks = std::max(1, (nj + g6k::kTJ - 1) / g6k::kTJ);
+    const int base = (2048 + iblocks - 1) / iblocks;
+    const int deep = std::min((32768 + iblocks - 1) / iblocks, nj / (16 * g6k::kTJ));
+    int nchunk = std::min(std::max(1, std::max(base, deep)), max_chunks);
+    while (nchunk > 1 && (long long)nchunk * ni > kPartBudget) nchunk /= 2;
+    return nchunk;
+}
+
 ForceCfg choose_cfg(int ni, int nj)
 {
     ForceCfg c;
@@ -365,32 +381,43 @@ ForceCfg choose_cfg(int ni, int nj)
         njs = 64; ipl = 1; nchunk = 1; smem = 0;
     }
     if (njs == 0) {
-        // Largest i-block that does not leave lanes idle, then enough
-        // j-chunks for ~8 workgroups per CU on 256 CUs.
-        static const int kOrder[][2] = {{1, 2}, {1, 1}, {2, 1}, {4, 1}, {8, 1}, {16, 1},
-                                        {32, 1}, {64, 1}, {128, 1}, {256, 1}};
         int p2 = 1;
         while (p2 < ni) p2 *= 2;
-        for (const auto &o : kOrder) {
-            const int ib = (g6k::kWG / o[0]) * o[1];
-            if (ib <= p2) { njs = o[0]; ipl = o[1]; break; }
+        if ((long long)ni * nj <= (1ll << 29)) {
+            // latency-bound launches: the widest i-block of at most 64 i (4
+            // lanes per i) that still gives >= 256 workgroups with the
+            // j-chunks the range allows, else the one with the most (best
+            // or within a few % of the best of eleven shapes at nj = 16384
+            // and 65536 for every ni = 2^k below 2^29 / nj, both
+            // precisions: wider i-blocks leave too few workgroups once the
+            // j-range is split as far as it goes, narrower ones pay for
+            // their cross-lane sums and dispatch)
+            long long most = -1;
+            for (int ib = std::min(64, p2); ib >= 1; ib /= 2) {
+                const int iblocks = (ni + ib - 1) / ib;
+                const long long items = (long long)iblocks * auto_chunks(iblocks, ni, nj);
+                if (items > most) { most = items; njs = g6k::kWG / ib; }
+                if (items >= 256) break;
+            }
+            ipl = 1;
+        } else {
+            // throughput-bound: the widest i-block that does not leave lanes
+            // idle, the j-range split into enough chunks below
+            static const int kOrder[][2] = {{1, 2}, {1, 1}, {2, 1}, {4, 1}, {8, 1}, {16, 1},
+                                            {32, 1}, {64, 1}, {128, 1}, {256, 1}};
+            for (const auto &o : kOrder) {
+                const int ib = (g6k::kWG / o[0]) * o[1];
+                if (ib <= p2) { njs = o[0]; ipl = o[1]; break; }
+            }
+            if (njs == 0) { njs = 256; ipl = 1; }
         }
-        if (njs == 0) { njs = 256; ipl = 1; }
         nchunk = 0;
         smem = 0;  // LDS tiles measured faster than scalar loads at every size tried
     }
     const int ib = (g6k::kWG / njs) * ipl;
     const int iblocks = (ni + ib - 1) / ib;
     const int max_chunks = std::max(1, (nj + g6k::kTJ - 1) / g6k::kTJ);
-    if (nchunk <= 0) {
-        // ~8 workgroups per CU at least; for long j-ranges up to ~128, so a
-        // launch is many rounds of workgroups and the last round's imbalance
-        // is small (at ni = nj = 2^20, two rounds lost 5 % against 32), with
-        // every chunk still >= 16 tiles
-        const int base = (2048 + iblocks - 1) / iblocks;
-        const int deep = std::min((32768 + iblocks - 1) / iblocks, nj / (16 * g6k::kTJ));
-        nchunk = std::max(1, std::max(base, deep));
-    }
+    if (nchunk <= 0) nchunk = auto_chunks(iblocks, ni, nj);
     nchunk = std::min(nchunk, max_chunks);
     while (nchunk > 1 && (long long)nchunk * ni > kPartBudget) nchunk /= 2;
     int jchunk = (nj + nchunk - 1) / nchunk;

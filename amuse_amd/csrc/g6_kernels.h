@@ -138,10 +138,10 @@ __host__ __device__ inline int step_njs(int shape)
 // Launch plan of one step.  Small systems (nj <= 2048, at most 8 tiles):
 // 64 lanes per i and no j-split over workgroups, so a step is one short
 // force kernel and no reduce.  Otherwise the rule of the ABI path's
-// automatic choice: the widest i-block not wider than the next power of two
-// of ni, then j-chunks for ~kStepItems work items within the partials budget
-// (measured better than 64 lanes per i from nj = 16384 on: with 4 i per
-// workgroup the j-tiles are loaded once per 4 i and the loads dominate).
+// automatic choice: while ni * nj <= 2^29 (latency-bound) the widest
+// i-block of at most 64 i that gives >= 256 workgroups; beyond, the widest
+// i-block not wider than the next power of two of ni; then j-chunks for
+// ~kStepItems work items within the partials budget.
 __host__ __device__ inline StepPlan plan_step(int ni, int nj)
 {
     StepPlan sp;
@@ -155,11 +155,25 @@ __host__ __device__ inline StepPlan plan_step(int ni, int nj)
     int p2 = 1;
     while (p2 < ni) p2 *= 2;
     sp.shape = 3;
-    for (int k = 0; k < 4; k++)
-        if (step_ib(k) <= p2) { sp.shape = k; break; }
+    const int max_chunks = (nj + kTJ - 1) / kTJ;
+    if ((long long)ni * nj <= (1ll << 29)) {
+        // latency-bound: the widest of this kernel's i-blocks of at most 64
+        // i that gives >= 256 workgroups with the chunks the j-range allows
+        // (the library's automatic choice on this kernel's shapes)
+        for (int k = 1; k < 4; k++) {
+            sp.shape = k;
+            const int ib = step_ib(k);
+            if (ib > p2 && k < 3) continue;
+            const long long iblocks = (ni + ib - 1) / ib;
+            const long long want = (kStepItems + iblocks - 1) / iblocks;
+            if (iblocks * (want < max_chunks ? want : max_chunks) >= 256) break;
+        }
+    } else {
+        for (int k = 0; k < 4; k++)
+            if (step_ib(k) <= p2) { sp.shape = k; break; }
+    }
     const int ib = step_ib(sp.shape);
     sp.iblocks = (ni + ib - 1) / ib;
-    const int max_chunks = (nj + kTJ - 1) / kTJ;
     int nchunk = sp.iblocks > 0 ? (kStepItems + sp.iblocks - 1) / sp.iblocks : 1;
     if (nchunk > max_chunks) nchunk = max_chunks;
     while (nchunk > 1 && (long long)nchunk * ni > kStepPartBudget) nchunk /= 2;
