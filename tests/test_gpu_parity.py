@@ -225,6 +225,35 @@ def test_tile_shapes_agree(g6):
     g6.set_tiles(0, 0)
 
 
+def test_host_io_boundaries(g6):
+    """i-blocks and j-stages on both sides of the 4096-row limit below which
+    the kernels read inputs and write results in pinned host memory
+    themselves (above it the rows are copied), and a launch whose j-split
+    would read each i-row too often in place (its rows are copied)."""
+    n = 6000
+    s = plummer_state(n, 27)
+    # the first 4096 slots go in wrong (a 6000-record stage, copied), then
+    # right (a 4096-record stage, read in place by the scatter kernel)
+    wrong = s["x"].copy()
+    wrong[:4096] += 0.25
+    upload(g6, wrong, s["v"], s["a2"], s["j6"], s["k18"], s["m"], s["tj"], s["ids"])
+    g6.set_ti(s["ti"])
+    g6.calc(n, s["ids"][:1], s["x"][:1], s["v"][:1], 1e-4)  # predicted at ti
+    sel = np.arange(4096)
+    g6.set_j_particles(sel, s["ids"][sel], s["tj"][sel], 0.0, s["m"][sel], s["k18"][sel],
+                       s["j6"][sel], s["a2"][sel], s["v"][sel], s["x"][sel])
+    xp, vp = oracle.predict(s["ti"], s["tj"], s["x"], s["v"], s["a2"], s["j6"], s["k18"])
+    for nj, ni in ((1024, 4096), (1024, 4097), (n, 64), (n, 2000), (n, 4097)):
+        print_case(f"host i/o nj={nj} ni={ni}")
+        isel = np.linspace(0, n - 1, ni).astype(int)
+        id_i, xi, vi = s["ids"][isel], xp[isel], vp[isel]
+        acc, jerk, pot, nn, _ = g6.calc(nj, id_i, xi, vi, 1e-4)
+        jid, jm, jx, jv = s["ids"][:nj], s["m"][:nj], xp[:nj], vp[:nj]
+        _, _, _, onn, _ = oracle.force(id_i, xi, vi, 1e-4, jid, jm, jx, jv)
+        assert np.array_equal(nn, onn), (nj, ni)
+        assert_plain_parity(acc, jerk, pot, oracle.force_ld(id_i, xi, vi, 1e-4, jid, jm, jx, jv))
+
+
 def test_many_chunk_reduction(g6):
     """Small i-blocks against many j split over 16..79 chunks: the
     one-wave-per-i reduction keeps FP64 parity and the exact nn."""
