@@ -100,6 +100,8 @@ struct Unit {
     std::vector<int> l_id;
     bool launched = false;
     bool launched_nn = false;
+    bool l_pred = false;  // the latched call's force kernel predicts its j-slots
+    double l_ti = 0.0;    // ti of the latched call
     int last_ni = 0;  // rows valid in ib.out
 
     int tile_njs = 0, tile_ipl = 0, tile_nchunk = 0, tile_smem = 0;
@@ -327,9 +329,16 @@ int flush_stage()
     return 0;
 }
 
+// The predicted store holds the j-slots at ti for the current precision and
+// predictor mode.
+bool pred_current()
+{
+    return U.pred_valid && U.pred_ti == U.ti && U.pred_prec == U.prec && U.pred_compat == U.compat;
+}
+
 int ensure_predicted()
 {
-    if (U.pred_valid && U.pred_ti == U.ti && U.pred_prec == U.prec && U.pred_compat == U.compat) return 0;
+    if (pred_current()) return 0;
     Unit::Ev ev;
     timed_begin(ev, 2, 0.0);
     HIPCHK(g6k::launch_predict(U.js, U.nj_hw, U.ti, U.prec, U.compat, U.k18_nonzero ? 1 : 0, U.stream), "predict");
@@ -470,10 +479,22 @@ int run_force(int ni, int nj, double eps2, bool want_nn)
     ForceCfg cfg = choose_cfg(ni, std::max(nj, 1));
     int rc = ensure_partials(cfg.nchunk, ni);
     if (rc) return rc;
+    // the latched call's j-slots predicted by the force kernel itself (see
+    // do_firsthalf), unless the shape changed since: then by the predictor
+    bool pred = false;
+    if (U.latched && U.l_pred) {
+        pred = g6k::pred_force_cfg(cfg, nj);
+        if (!pred) {
+            rc = ensure_predicted();
+            if (rc) return rc;
+        }
+    }
     const IBlock b = io_view(ni);
     Unit::Ev ev;
     timed_begin(ev, 0, double(ni) * double(nj));
-    HIPCHK(g6k::launch_force(U.js, b, ni, nj, eps2, U.prec, want_nn, cfg, U.stream), "force kernel");
+    HIPCHK(g6k::launch_force(U.js, b, ni, nj, eps2, U.prec, want_nn, cfg, U.stream, pred, U.l_ti,
+                             U.compat),
+           "force kernel");
     timed_end(ev);
     if (cfg.nchunk > 1) {
         Unit::Ev ev2;
@@ -598,8 +619,15 @@ int do_firsthalf(int nj, int ni, const int *index, const double (*xi)[3], const 
     if (rc) return rc;
     rc = flush_stage();
     if (rc) return rc;
-    rc = ensure_predicted();
-    if (rc) return rc;
+    const ForceCfg cfg = choose_cfg(std::max(ni, 1), std::max(nj, 1));
+    // a short j-range's force kernel predicts the j-slots as it loads them
+    // (no predictor launch); otherwise the store is predicted to ti here
+    U.l_pred = ni > 0 && g6k::pred_force_cfg(cfg, nj) && !U.k18_nonzero && !pred_current();
+    U.l_ti = U.ti;
+    if (!U.l_pred) {
+        rc = ensure_predicted();
+        if (rc) return rc;
+    }
     if (ni > 0) {
         if (U.ev_in) HIPCHK(hipEventSynchronize(U.ev_in), "i event");
         const size_t C = size_t(U.ib.cap);
@@ -614,7 +642,6 @@ int do_firsthalf(int nj, int ni, const int *index, const double (*xi)[3], const 
         }
         // read in place when the force launch reads each row a few times at
         // most (once per j-chunk of its i-block), else copied once
-        const ForceCfg cfg = choose_cfg(ni, std::max(nj, 1));
         U.in_direct = (long long)cfg.nchunk * ni <= kDirectRows;
         if (!U.in_direct) {
             // all eight rows in one pitched copy (the id row's second half
@@ -780,6 +807,8 @@ G6_API int g6_read_neighbour_list_(int *)
         U.ngb_cap_i = cap;
     }
     HIPCHK(hipMemsetAsync(U.d_ngb_counts, 0, sizeof(int) * size_t(ni), U.stream), "ngb clear");
+    rc = ensure_predicted();  // the force kernel may have predicted on the fly
+    if (rc) return rc;
     if (U.in_direct) {
         // the query reads each i once per j-tile: the rows go to the device
         const size_t C = size_t(U.ib.cap);

@@ -215,9 +215,19 @@ hipError_t launch_iblock_self(const JStore &s, const IBlock &b, int i0,
                               int ni, int prec, hipStream_t st);
 // Valid (njs, ipl) pairs compiled in.
 bool force_cfg_supported(int njs, int ipl);
+// pred_j: the j-slots are predicted to ti (compat mode as given, k18 = 0)
+// from their base records as the kernel loads them, and the predicted store
+// is neither read nor written; only for pred_force_cfg launches.
 hipError_t launch_force(const JStore &s, const IBlock &b, int ni, int nj,
                         double eps2, int prec, bool want_nn,
-                        const ForceCfg &cfg, hipStream_t st);
+                        const ForceCfg &cfg, hipStream_t st, bool pred_j = false,
+                        double ti = 0.0, int compat = 0);
+// Launches that can predict their j-slots as they load them: one i per
+// wave (64 lanes), one j-chunk, at most 8 tiles.
+inline bool pred_force_cfg(const ForceCfg &c, int nj)
+{
+    return c.njs == 64 && c.ipl == 1 && c.nchunk == 1 && !c.smem && nj <= 8 * kTJ;
+}
 hipError_t launch_reduce(const JStore &s, const IBlock &b, int ni,
                          int nchunk, int prec, bool want_nn, hipStream_t st);
 // Neighbour query over the latched i-block: ids of j with unsoftened

@@ -294,7 +294,25 @@ struct FParams {
     int ostride;
     double *part;  // [chunk][fields][pstride]
     int pstride;
+    // j-slots predicted as tiles are loaded (items with PRED): the base
+    // records and the time they are predicted to
+    const double *bx[3], *bv[3], *ba2[3], *bj6[3], *btj;
+    double tpred;
+    int compat;
 };
+
+// j-slot j predicted to p.tpred from its base record, as the predictor
+// kernels do (k18 = 0); the caller converts it as write_pred does.
+__device__ __forceinline__ Pred predict_base(const FParams &p, int j)
+{
+    double x[3], v[3], a2[3], j6[3];
+    const double k18[3] = {0.0, 0.0, 0.0};
+#pragma unroll
+    for (int k = 0; k < 3; k++) {
+        x[k] = p.bx[k][j]; v[k] = p.bv[k][j]; a2[k] = p.ba2[k][j]; j6[k] = p.bj6[k][j];
+    }
+    return predict_slot(p.tpred, p.btj[j], x, v, a2, j6, k18, p.compat, false);
+}
 
 // Shared-memory layout of one j tile.
 struct TileD {
@@ -308,7 +326,9 @@ struct TileF {
 
 // One (i-block bx, j-chunk by) work item; the kernel below runs one item
 // per workgroup, kd_* below loops over items whose count lives on the device.
-template <int NJS, int IPL, bool NN>
+// PRED: the j-slots are predicted to p.tpred as the tiles are loaded (the
+// values write_pred would store), not read from the predicted store.
+template <int NJS, int IPL, bool NN, bool PRED = false>
 __device__ __forceinline__ void force_f64_item(const FParams &p, TileD &tile, int bx, int by,
                                        int nchunk)
 {
@@ -345,10 +365,20 @@ __device__ __forceinline__ void force_f64_item(const FParams &p, TileD &tile, in
         __syncthreads();
         if (t < jn) {
             const int j = tb + t;
-            tile.xy[t] = make_double2(p.jx[j], p.jy[j]);
-            tile.zm[t] = make_double2(p.jz[j], p.jm[j]);
-            tile.vxy[t] = make_double2(p.jvx[j], p.jvy[j]);
-            tile.vzid[t] = make_double2(p.jvz[j], __longlong_as_double((long long)p.jid[j]));
+            if constexpr (PRED) {
+                const Pred pr = predict_base(p, j);
+                const bool empty = p.jid[j] == -1;
+                tile.xy[t] = make_double2(empty ? kFar : pr.x[0], empty ? kFar : pr.x[1]);
+                tile.zm[t] = make_double2(empty ? kFar : pr.x[2], p.jm[j]);
+                tile.vxy[t] = make_double2(empty ? 0.0 : pr.v[0], empty ? 0.0 : pr.v[1]);
+                tile.vzid[t] = make_double2(empty ? 0.0 : pr.v[2],
+                                            __longlong_as_double((long long)p.jid[j]));
+            } else {
+                tile.xy[t] = make_double2(p.jx[j], p.jy[j]);
+                tile.zm[t] = make_double2(p.jz[j], p.jm[j]);
+                tile.vxy[t] = make_double2(p.jvx[j], p.jvy[j]);
+                tile.vzid[t] = make_double2(p.jvz[j], __longlong_as_double((long long)p.jid[j]));
+            }
         }
         __syncthreads();
 #pragma unroll G6_F64_UNROLL
@@ -451,6 +481,15 @@ __global__ void __launch_bounds__(kWG) k_force_f64(FParams p)
     force_f64_item<NJS, IPL, NN>(p, tile, blockIdx.x, blockIdx.y, gridDim.y);
 }
 
+// Short j-ranges with the j-slots predicted as they are loaded (one i per
+// wave, one chunk): the step of a small system needs no predictor launch.
+template <bool NN>
+__global__ void __launch_bounds__(kWG) k_force_f64_p(FParams p)
+{
+    __shared__ TileD tile;
+    force_f64_item<64, 1, NN, true>(p, tile, blockIdx.x, 0, 1);
+}
+
 // ------------------------------------------------------------------------
 // Pair interaction, mixed precision.
 // ------------------------------------------------------------------------
@@ -551,7 +590,8 @@ constexpr int kLongItem = 16384;
 // found once at the end; otherwise a compare and two selects per pair, as a
 // final search (up to kFlush dependent loads) would dominate a short item.
 // Both give the first slot with the lane's minimal r^2.
-template <int NJS, int IPL, bool NN, bool LONG>
+// PRED: as in force_f64_item.
+template <int NJS, int IPL, bool NN, bool LONG, bool PRED = false>
 __device__ __forceinline__ void force_mx_item(const FParams &p, TileF &tile, int bx, int by,
                                        int nchunk)
 {
@@ -589,9 +629,25 @@ __device__ __forceinline__ void force_mx_item(const FParams &p, TileF &tile, int
         __syncthreads();
         if (t < jn) {
             const int j = tb + t;
-            tile.a[t] = make_float4(p.hx[j], p.hy[j], p.hz[j], p.fm[j]);
-            tile.b[t] = make_float4(p.lx[j], p.ly[j], p.lz[j], p.fvx[j]);
-            tile.c[t] = make_float4(p.fvy[j], p.fvz[j], __int_as_float(p.jid[j]), 0.0f);
+            if constexpr (PRED) {
+                const Pred pr = predict_base(p, j);
+                const bool empty = p.jid[j] == -1;
+                float hi[3], lo[3], fv[3];
+#pragma unroll
+                for (int k = 0; k < 3; k++) {
+                    const float h = float(pr.x[k]);
+                    hi[k] = empty ? kFarF : h;
+                    lo[k] = empty ? 0.0f : float(pr.x[k] - double(h));
+                    fv[k] = empty ? 0.0f : float(pr.v[k]);
+                }
+                tile.a[t] = make_float4(hi[0], hi[1], hi[2], p.fm[j]);
+                tile.b[t] = make_float4(lo[0], lo[1], lo[2], fv[0]);
+                tile.c[t] = make_float4(fv[1], fv[2], __int_as_float(p.jid[j]), 0.0f);
+            } else {
+                tile.a[t] = make_float4(p.hx[j], p.hy[j], p.hz[j], p.fm[j]);
+                tile.b[t] = make_float4(p.lx[j], p.ly[j], p.lz[j], p.fvx[j]);
+                tile.c[t] = make_float4(p.fvy[j], p.fvz[j], __int_as_float(p.jid[j]), 0.0f);
+            }
         }
         __syncthreads();
         for (int jb = js; jb < jn; jb += NJS * kFlush) {
@@ -723,6 +779,13 @@ __device__ __forceinline__ void force_mx_item(const FParams &p, TileF &tile, int
             r[8 * PS + i] = make_float2(__int_as_float(NN ? A[q].addr : -1), 0.0f);
         }
     }
+}
+
+template <bool NN>
+__global__ void __launch_bounds__(kWG) k_force_mx_p(FParams p)
+{
+    __shared__ TileF tile;
+    force_mx_item<64, 1, NN, false, true>(p, tile, blockIdx.x, 0, 1);
 }
 
 template <int NJS, int IPL, bool NN>
@@ -1124,7 +1187,8 @@ static void launch_force_t(const FParams &fp, int prec, bool nn, dim3 grid, hipS
 }
 
 hipError_t launch_force(const JStore &s, const IBlock &b, int ni, int nj, double eps2, int prec,
-                        bool want_nn, const ForceCfg &cfg, hipStream_t st)
+                        bool want_nn, const ForceCfg &cfg, hipStream_t st, bool pred_j,
+                        double ti, int compat)
 {
     if (ni <= 0) return hipSuccess;
     FParams fp{};
@@ -1148,6 +1212,23 @@ hipError_t launch_force(const JStore &s, const IBlock &b, int ni, int nj, double
     fp.part = b.part;
     const int ib = (kWG / cfg.njs) * cfg.ipl;
     dim3 grid(nblocks(ni, ib), cfg.nchunk);
+    if (pred_j) {
+        if (!pred_force_cfg(cfg, nj)) return hipErrorInvalidValue;
+        for (int k = 0; k < 3; k++) {
+            fp.bx[k] = s.x[k]; fp.bv[k] = s.v[k]; fp.ba2[k] = s.a2[k]; fp.bj6[k] = s.j6[k];
+        }
+        fp.btj = s.tj;
+        fp.tpred = ti;
+        fp.compat = compat;
+        if (prec == kF64) {
+            if (want_nn) hipLaunchKernelGGL((k_force_f64_p<true>), grid, dim3(kWG), 0, st, fp);
+            else hipLaunchKernelGGL((k_force_f64_p<false>), grid, dim3(kWG), 0, st, fp);
+        } else {
+            if (want_nn) hipLaunchKernelGGL((k_force_mx_p<true>), grid, dim3(kWG), 0, st, fp);
+            else hipLaunchKernelGGL((k_force_mx_p<false>), grid, dim3(kWG), 0, st, fp);
+        }
+        return hipGetLastError();
+    }
     if (cfg.smem) {
         if (cfg.njs != 1) return hipErrorInvalidValue;
         if (cfg.ipl == 1) launch_force_s<1>(fp, prec, want_nn, grid, st);
