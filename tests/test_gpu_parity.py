@@ -297,6 +297,36 @@ def test_self_block_matches_abi_path(g6):
     np.testing.assert_array_equal(pot, pot2)
 
 
+@pytest.mark.parametrize("prec,compat", [(0, False), (0, True), (1, False), (1, True)])
+def test_predicted_on_load_matches_predictor(g6, prec, compat):
+    """Short j-ranges: the ABI call's force kernel predicts the j-slots as it
+    loads them; the same i-block through g6x_calc_self (the predictor
+    kernel, then the same force shape) gives the same bits, nn included."""
+    nj = 777  # a ragged last tile
+    s = plummer_state(nj, 28)
+    g6.set_precision(prec)
+    g6.set_predictor_compat(compat)
+    upload(g6, s["x"], s["v"], s["a2"], s["j6"], s["k18"], s["m"], s["tj"], s["ids"])
+    g6.set_ti(s["ti"])
+    xp, vp = oracle.predict(s["ti"], s["tj"], s["x"], s["v"], s["a2"], s["j6"], s["k18"],
+                            compat=compat)
+    lo, ni = 100, 300
+    print_case(f"predicted on load prec={prec} compat={compat}")
+    acc, jerk, pot, nn, _ = g6.calc(nj, s["ids"][lo:lo + ni], xp[lo:lo + ni], vp[lo:lo + ni], 1e-4)
+    g6.calc_self(nj, lo, ni, 1e-4, nn=True)
+    acc2, jerk2, pot2, nn2 = g6.fetch(ni)
+    g6.set_predictor_compat(False)
+    g6.set_precision(0)
+    assert np.array_equal(nn, nn2)
+    np.testing.assert_array_equal(acc, acc2)
+    np.testing.assert_array_equal(jerk, jerk2)
+    np.testing.assert_array_equal(pot, pot2)
+    if prec == 0:
+        _, _, _, onn, _ = oracle.force(s["ids"][lo:lo + ni], xp[lo:lo + ni], vp[lo:lo + ni],
+                                       1e-4, s["ids"], s["m"], xp, vp)
+        assert np.array_equal(nn, onn)
+
+
 def test_deterministic_repeat(g6):
     s = plummer_state(5000, 24)
     upload(g6, s["x"], s["v"], s["a2"], s["j6"], s["k18"], s["m"], s["tj"], s["ids"])
