@@ -34,6 +34,12 @@ constexpr double kFar = 1.0e150;  // position given to empty slots
 // in flight to hide the serial chunk loop).
 constexpr int kReduceWaveMin = 16;
 constexpr int kReduceWaveMaxNi = 8192;
+// From kReduceRowMin chunks up to kReduceRowMaxNi rows the library's
+// reduce kernel takes a whole workgroup per row instead (ABI path; at
+// nj = 65536 force + fold -14..-20 % FP64 and -5..-11 % mixed for ni =
+// 4-128, while from ni = 512 the wave form's ni / 4 workgroups were faster).
+constexpr int kReduceRowMin = 128;
+constexpr int kReduceRowMaxNi = 128;
 constexpr int kNgbCap = 256;      // neighbour ids kept per i (sapporo NGB_PP)
 
 struct JStore {

@@ -955,6 +955,70 @@ __device__ __forceinline__ void reduce_one(const double *__restrict__ part, int 
     }
 }
 
+// Row form for many chunks and few rows: one row per workgroup, the 256
+// lanes take every 256th chunk, each wave's sums meet in its lane 0 by the
+// butterfly and the four waves' in wave order through LDS (a fixed order, so
+// runs repeat bit for bit).  The wave form puts one wave per row on ni / 4
+// workgroups, too few CUs to stream the many chunk records of a small
+// block.
+template <bool MX>
+__global__ void __launch_bounds__(256) k_reduce_row(const double *__restrict__ part, int nchunk,
+                                                    int PS, int S, const int *__restrict__ jid,
+                                                    double *__restrict__ out,
+                                                    int *__restrict__ nn, int want_nn)
+{
+    const int i = blockIdx.x;
+    const int t = threadIdx.x, lane = t & 63, wave = t >> 6;
+    int addr;
+    if constexpr (MX) {
+        float h[7], l[7], best;
+        reduce_i_mx(part, nchunk, PS, i, t, 256, h, l, best, addr, want_nn);
+        butterfly_mx(h, l, best, addr);
+        __shared__ float sh[4][7], sl[4][7], sb[4];
+        __shared__ int sa[4];
+        if (lane == 0) {
+#pragma unroll
+            for (int k = 0; k < 7; k++) { sh[wave][k] = h[k]; sl[wave][k] = l[k]; }
+            sb[wave] = best;
+            sa[wave] = addr;
+        }
+        __syncthreads();
+        if (t != 0) return;
+        for (int w = 1; w < 4; w++) {
+#pragma unroll
+            for (int k = 0; k < 7; k++) {
+                two_sum_add(h[k], l[k], sh[w][k]);
+                l[k] += sl[w][k];
+            }
+            nn_merge_f(best, addr, sb[w], sa[w]);
+        }
+        store_reduced(h, l, addr, S, i, jid, out, nn, want_nn);
+    } else {
+        double h[7], l[7], best;
+        reduce_i_f64(part, nchunk, PS, i, t, 256, h, l, best, addr, want_nn);
+        butterfly_f64(h, l, best, addr);
+        __shared__ double sh[4][7], sl[4][7], sb[4];
+        __shared__ int sa[4];
+        if (lane == 0) {
+#pragma unroll
+            for (int k = 0; k < 7; k++) { sh[wave][k] = h[k]; sl[wave][k] = l[k]; }
+            sb[wave] = best;
+            sa[wave] = addr;
+        }
+        __syncthreads();
+        if (t != 0) return;
+        for (int w = 1; w < 4; w++) {
+#pragma unroll
+            for (int k = 0; k < 7; k++) {
+                two_sum_add_d(h[k], l[k], sh[w][k]);
+                l[k] += sl[w][k];
+            }
+            nn_merge(best, addr, sb[w], sa[w]);
+        }
+        store_reduced(h, l, addr, S, i, jid, out, nn, want_nn);
+    }
+}
+
 template <bool MX, bool WAVE>
 __global__ void __launch_bounds__(256) k_reduce(const double *__restrict__ part, int nchunk,
                                                 int ni, int PS, int S,
@@ -1263,6 +1327,15 @@ hipError_t launch_reduce(const JStore &s, const IBlock &b, int ni, int nchunk, i
 {
     if (ni <= 0 || nchunk <= 1) return hipSuccess;
     const int S = b.cap;
+    if (nchunk >= kReduceRowMin && ni <= kReduceRowMaxNi) {
+        if (prec == kF64)
+            hipLaunchKernelGGL((k_reduce_row<false>), dim3(ni), dim3(256), 0, st, b.part, nchunk,
+                               ni, S, s.pid, b.out, b.nn, int(want_nn));
+        else
+            hipLaunchKernelGGL((k_reduce_row<true>), dim3(ni), dim3(256), 0, st, b.part, nchunk,
+                               ni, S, s.pid, b.out, b.nn, int(want_nn));
+        return hipGetLastError();
+    }
     if (nchunk >= kReduceWaveMin && ni <= kReduceWaveMaxNi) {
         if (prec == kF64)
             hipLaunchKernelGGL((k_reduce<false, true>), dim3(nblocks(ni, 4)), dim3(256), 0, st,

@@ -283,6 +283,39 @@ def test_many_chunk_reduction(g6):
     g6.set_precision(0)
 
 
+def test_row_form_reduction(g6):
+    """Small i-blocks against 40000 j: 157 one-tile chunks, folded by the
+    reduce kernel's row form (one workgroup per row; from 128 chunks up to
+    128 rows; 300 rows take the wave form); 100 forced chunks take the wave
+    form.  FP64 parity and the exact nn either way; mixed within its
+    bound."""
+    n = 40000
+    s = plummer_state(n, 29)
+    upload(g6, s["x"], s["v"], s["a2"], s["j6"], s["k18"], s["m"], s["tj"], s["ids"])
+    g6.set_ti(s["ti"])
+    xp, vp = oracle.predict(s["ti"], s["tj"], s["x"], s["v"], s["a2"], s["j6"], s["k18"])
+    for ni in (1, 7, 64, 300):
+        sel = np.arange(0, n, n // ni)[:ni]
+        id_i, xi, vi = s["ids"][sel], xp[sel], vp[sel]
+        exact = oracle.force_ld(id_i, xi, vi, 1e-4, s["ids"], s["m"], xp, vp)
+        _, _, _, onn, _ = oracle.force(id_i, xi, vi, 1e-4, s["ids"], s["m"], xp, vp)
+        for prec in (0, 1):
+            g6.set_precision(prec)
+            for nchunk in (0, 100):
+                print_case(f"row form ni={ni} prec={prec} nchunk={nchunk}")
+                if nchunk:
+                    g6.set_tiles(256 if ni == 1 else 64, 1, nchunk, 0)
+                acc, jerk, pot, nn, _ = g6.calc(n, id_i, xi, vi, 1e-4)
+                g6.set_tiles(0, 0)
+                if prec == 0:
+                    assert np.array_equal(nn, onn), (ni, nchunk)
+                    assert_plain_parity(acc, jerk, pot, exact)
+                else:
+                    d = np.abs(acc - exact[0]).max(1)
+                    assert (d <= 2e-6 * exact[3]).all(), (ni, nchunk)
+    g6.set_precision(0)
+
+
 def test_self_block_matches_abi_path(g6):
     s = plummer_state(2048, 23)
     upload(g6, s["x"], s["v"], s["a2"], s["j6"], s["k18"], s["m"], s["tj"], s["ids"])
