@@ -329,22 +329,23 @@ int flush_stage()
     return 0;
 }
 
-// The predicted store holds the j-slots at ti for the current precision and
-// predictor mode.
-bool pred_current()
+// The predicted store holds the j-slots at time t for the current precision
+// and predictor mode.  A latched calc and its neighbour query use the ti of
+// its firsthalf (l_ti), whatever set_ti was called with since.
+bool pred_current(double t)
 {
-    return U.pred_valid && U.pred_ti == U.ti && U.pred_prec == U.prec && U.pred_compat == U.compat;
+    return U.pred_valid && U.pred_ti == t && U.pred_prec == U.prec && U.pred_compat == U.compat;
 }
 
-int ensure_predicted()
+int ensure_predicted(double t)
 {
-    if (pred_current()) return 0;
+    if (pred_current(t)) return 0;
     Unit::Ev ev;
     timed_begin(ev, 2, 0.0);
-    HIPCHK(g6k::launch_predict(U.js, U.nj_hw, U.ti, U.prec, U.compat, U.k18_nonzero ? 1 : 0, U.stream), "predict");
+    HIPCHK(g6k::launch_predict(U.js, U.nj_hw, t, U.prec, U.compat, U.k18_nonzero ? 1 : 0, U.stream), "predict");
     timed_end(ev);
     U.pred_valid = true;
-    U.pred_ti = U.ti;
+    U.pred_ti = t;
     U.pred_prec = U.prec;
     U.pred_compat = U.compat;
     return 0;
@@ -485,7 +486,7 @@ int run_force(int ni, int nj, double eps2, bool want_nn)
     if (U.latched && U.l_pred) {
         pred = g6k::pred_force_cfg(cfg, nj);
         if (!pred) {
-            rc = ensure_predicted();
+            rc = ensure_predicted(U.l_ti);
             if (rc) return rc;
         }
     }
@@ -622,10 +623,10 @@ int do_firsthalf(int nj, int ni, const int *index, const double (*xi)[3], const 
     const ForceCfg cfg = choose_cfg(std::max(ni, 1), std::max(nj, 1));
     // a short j-range's force kernel predicts the j-slots as it loads them
     // (no predictor launch); otherwise the store is predicted to ti here
-    U.l_pred = ni > 0 && g6k::pred_force_cfg(cfg, nj) && !U.k18_nonzero && !pred_current();
+    U.l_pred = ni > 0 && g6k::pred_force_cfg(cfg, nj) && !U.k18_nonzero && !pred_current(U.ti);
     U.l_ti = U.ti;
     if (!U.l_pred) {
-        rc = ensure_predicted();
+        rc = ensure_predicted(U.l_ti);
         if (rc) return rc;
     }
     if (ni > 0) {
@@ -807,7 +808,7 @@ G6_API int g6_read_neighbour_list_(int *)
         U.ngb_cap_i = cap;
     }
     HIPCHK(hipMemsetAsync(U.d_ngb_counts, 0, sizeof(int) * size_t(ni), U.stream), "ngb clear");
-    rc = ensure_predicted();  // the force kernel may have predicted on the fly
+    rc = ensure_predicted(U.l_ti);  // the force kernel may have predicted on the fly
     if (rc) return rc;
     if (U.in_direct) {
         // the query reads each i once per j-tile: the rows go to the device
@@ -960,7 +961,7 @@ G6_API int g6x_calc_self(int nj, int i0, int ni, double eps2, int want_nn)
     if (rc) return rc;
     rc = flush_stage();
     if (rc) return rc;
-    rc = ensure_predicted();
+    rc = ensure_predicted(U.ti);
     if (rc) return rc;
     HIPCHK(g6k::launch_iblock_self(U.js, U.ib, i0, ni, U.prec, U.stream), "iblock self");
     U.latched = false;

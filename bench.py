@@ -85,6 +85,48 @@ def cpu_baseline(m, pos, vel, eps2: float, threads: int, per_thread: int):
     return sec, ni, (acc, jerk, pot, nn)
 
 
+def _read(path: str) -> str | None:
+    try:
+        with open(path) as f:
+            return f.read()
+    except OSError:
+        return None
+
+
+def host_info() -> dict:
+    """The host the CPU baselines run on: logical CPUs, the ones this process
+    may run on (affinity), the cgroup CPU quota, CPU model and clock."""
+    info = {"nproc": os.cpu_count(), "affinity": len(os.sched_getaffinity(0))}
+    quota = None
+    cg = _read("/sys/fs/cgroup/cpu.max")                   # cgroup v2: "quota period"
+    if cg and cg.split()[0] != "max":
+        quota = float(cg.split()[0]) / float(cg.split()[1])
+    else:
+        q = _read("/sys/fs/cgroup/cpu/cpu.cfs_quota_us")   # cgroup v1
+        p = _read("/sys/fs/cgroup/cpu/cpu.cfs_period_us")
+        if q and p and int(q) > 0:
+            quota = int(q) / int(p)
+    info["cgroup_cpu_quota"] = quota
+    cpuinfo = _read("/proc/cpuinfo") or ""
+    models = [ln.split(":", 1)[1].strip() for ln in cpuinfo.splitlines()
+              if ln.startswith("model name")]
+    mhz = [float(ln.split(":", 1)[1]) for ln in cpuinfo.splitlines() if ln.startswith("cpu MHz")]
+    info["cpu_model"] = models[0] if models else None
+    info["cpu_mhz_now_mean"] = round(sum(mhz) / len(mhz), 1) if mhz else None
+    fmax = _read("/sys/devices/system/cpu/cpu0/cpufreq/cpuinfo_max_freq")
+    info["cpu_mhz_max"] = int(fmax) / 1000 if fmax else None
+    return info
+
+
+def usable_cores(info: dict) -> int:
+    """Threads the CPU baselines use: every CPU of the affinity mask, capped
+    by the cgroup quota when one is set (threads beyond it are throttled)."""
+    n = info["affinity"]
+    if info["cgroup_cpu_quota"]:
+        n = min(n, max(1, int(info["cgroup_cpu_quota"])))
+    return n
+
+
 def load_traffic(kernel: str):
     path = os.path.join(ROOT, "profiles", "traffic.json")
     if not os.path.exists(path):
@@ -340,21 +382,26 @@ def c4_evolve(args) -> int:
     forces and once with mixed ones (initial forces and steps in FP64 for
     both); energies from FP64 passes.  Reports |dE/E| of each path and their
     ratio, block and particle steps, wall time and pair interactions/s."""
-    from amuse_amd.blocks import DeviceHermite4
     n, eps2 = args.n, args.eps2
     m, pos, vel = make_system(n)
-    res = {"workload": f"c4 evolve: N={n} Plummer/Salpeter, eps2={eps2}, eta={args.eta}, "
-                       f"t=0..{args.c4_t}, device-resident driver"}
+    print(json.dumps(c4_evolve_run(m, pos, vel, eps2, args.eta, args.c4_t)))
+    return 0
+
+
+def c4_evolve_run(m, pos, vel, eps2, eta, t_end, nout=16):
+    from amuse_amd.blocks import DeviceHermite4
+    n = m.size
+    res = {"workload": f"c4 evolve: N={n} Plummer/Salpeter, eps2={eps2}, eta={eta}, "
+                       f"t=0..{t_end}, device-resident driver"}
     for prec, name in ((0, "fp64"), (1, "mixed")):
         d = DeviceHermite4(precision=0)
-        d.commit(m, pos, vel, eps2=eps2, eta=args.eta)
+        d.commit(m, pos, vel, eps2=eps2, eta=eta)
         e0 = sum(d.energy())
         d.set_precision(prec)
         c0 = d.counters()
         t0 = time.perf_counter()
-        nout = 16
         for k in range(1, nout + 1):
-            d.evolve(args.c4_t * k / nout)
+            d.evolve(t_end * k / nout)
             c = d.counters()
             log(f"  {name}: t={c['system_time']:.6f} blocks={c['block_steps']} "
                 f"{time.perf_counter() - t0:.1f} s")
@@ -372,8 +419,7 @@ def c4_evolve(args) -> int:
     res["energy_error_ratio_mixed_over_fp64"] = abs(res["mixed"]["dE_over_E"]) / max(
         abs(res["fp64"]["dE_over_E"]), 1e-300)
     res["speed_ratio_mixed_over_fp64"] = res["fp64"]["wall_s"] / res["mixed"]["wall_s"]
-    print(json.dumps(res))
-    return 0
+    return res
 
 
 def worker_study(args) -> int:
@@ -649,10 +695,10 @@ def main() -> int:
     K = 0.5 * float((m * (vel * vel).sum(1)).sum())
 
     cpu = cpu_best = None
+    host = host_info()
     if not args.no_cpu:
-        import oracle
-        threads = args.cpu_threads or min(16, os.cpu_count() or 1)
-        log(f"bench: CPU baseline (g6lib restatement, {threads} threads)")
+        threads = args.cpu_threads or usable_cores(host)
+        log(f"bench: CPU baseline (g6lib restatement, {threads} threads; host {host})")
         sec, ni_cpu, cres = cpu_baseline(m, pos, vel, eps2, threads, args.cpu_per_thread)
         cpu_gps = ni_cpu * float(n) / sec / 1e9
         # the CPU sample's forces agree with the GPU FP64 path
@@ -662,10 +708,13 @@ def main() -> int:
                "sample": f"{ni_cpu} i-particles x {n} j (g6lib.c restatement: AoS records, "
                          f"pow() triple, per-call predictor, {args.cpu_per_thread} i per call), "
                          f"{threads} independent units on {threads} threads, {sec:.1f} s",
+               "host": host,
+               "per_core": cpu_gps / threads,
                "max_rel_acc_vs_gpu_fp64": float(dc.max())}
         # best-effort CPU row: ph4's own CPU pair loop, SIMD over j, threads over i
         ni_b = 128 * threads
         log(f"bench: best-effort CPU loop ({threads} threads, {ni_b} i)")
+        import oracle
         bsec, ba, _, _, _ = oracle.cpu_best_run(threads, pos[:ni_b], vel[:ni_b], eps2, m, pos, vel)
         db = np.linalg.norm(ba - a64[:ni_b], axis=1) / np.linalg.norm(a64[:ni_b], axis=1)
         cpu_best = {"value": ni_b * float(n) / bsec / 1e9, "unit": "G pair-interactions/s",
@@ -673,6 +722,7 @@ def main() -> int:
                     "sample": f"{ni_b} i-particles x {n} j, ph4's CPU pair loop "
                               f"(src/amuse_ph4/src/idata.cc:195-236) in FP64, AVX2 SIMD over j, "
                               f"OpenMP over i, {bsec:.1f} s",
+                    "per_core": ni_b * float(n) / bsec / 1e9 / threads,
                     "max_rel_acc_vs_gpu_fp64": float(db.max())}
 
     g6.close()
@@ -683,6 +733,12 @@ def main() -> int:
            "mixed": evolve_blocks(m, pos, vel, eps2, args.eta, 8, 1),
            "fp64_device_driver": evolve_blocks(m, pos, vel, eps2, args.eta, 8, 0, True),
            "mixed_device_driver": evolve_blocks(m, pos, vel, eps2, args.eta, 8, 1, True)}
+    # the discriminating energy evidence: the same system integrated to
+    # t = --c4-t (about 10^7 particle steps) with each path's forces
+    evolve = None
+    if args.c4_t > 0:
+        log(f"bench: integration to t={args.c4_t}, FP64 then mixed forces")
+        evolve = c4_evolve_run(m, pos, vel, eps2, args.eta, args.c4_t)
 
     traffic = load_traffic("mixed_full_n")
     out = {
@@ -720,19 +776,34 @@ def main() -> int:
         "cpu_baseline": cpu,
         "cpu_best_effort": cpu_best,
         "block_steps_8": evo,
-        "energy_error": {"fp64": evo["fp64"]["dE_over_E"], "mixed": evo["mixed"]["dE_over_E"],
-                         "eta": args.eta,
-                         "note": "relative energy change over 8 ph4 block steps after the "
-                                 "full-N evaluation (initial forces and steps in FP64 for "
-                                 "both), step forces in the named path, energies from FP64 "
-                                 "passes; the first blocks hold the particles with the "
-                                 "smallest steps, so the change is near the FP64 resolution "
-                                 "of the total energy"},
+        "energy_error": None if evolve is None else {
+            "fp64": evolve["fp64"]["dE_over_E"], "mixed": evolve["mixed"]["dE_over_E"],
+            "ratio_mixed_over_fp64": evolve["energy_error_ratio_mixed_over_fp64"],
+            "eta": args.eta, "t_end": args.c4_t,
+            "particle_steps": {"fp64": evolve["fp64"]["particle_steps"],
+                               "mixed": evolve["mixed"]["particle_steps"]},
+            "block_steps": {"fp64": evolve["fp64"]["block_steps"],
+                            "mixed": evolve["mixed"]["block_steps"]},
+            "gpairs_per_s": {"fp64": evolve["fp64"]["gpairs_per_s"],
+                             "mixed": evolve["mixed"]["gpairs_per_s"]},
+            "note": "relative energy change of the same N=2^20 system integrated from t=0 "
+                    "with ph4's block steps on the device-resident driver, step forces in "
+                    "the named path (initial forces and steps in FP64 for both), energies "
+                    "from FP64 passes"},
+        "energy_error_8_block_steps": {"fp64": evo["fp64"]["dE_over_E"],
+                                       "mixed": evo["mixed"]["dE_over_E"]},
     }
     if cpu:
         out["speedup_vs_cpu"] = {"mixed": gpmx / cpu["value"], "fp64": gp64 / cpu["value"]}
         out["speedup_vs_cpu_best_effort"] = {"mixed": gpmx / cpu_best["value"],
                                              "fp64": gp64 / cpu_best["value"]}
+        # every logical CPU of the host at the measured per-core rate: an upper
+        # bound on what the host could reach beyond the cores this run may use
+        nproc = host["nproc"] or cpu["cores"]
+        out["speedup_vs_cpu_best_effort_all_host_cpus"] = {
+            "cpus": nproc, "cpu_value": cpu_best["per_core"] * nproc,
+            "mixed": gpmx / (cpu_best["per_core"] * nproc),
+            "fp64": gp64 / (cpu_best["per_core"] * nproc)}
     print(json.dumps(out))
     return 0
 

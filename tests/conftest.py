@@ -45,3 +45,23 @@ def g6(g6lib):
 
 def print_case(msg):
     print(msg, flush=True)
+
+
+def oracle_threads():
+    """Threads for oracle sums in GPU tests: the CPUs this process may use,
+    at most 16 (one GPU's share of the host)."""
+    return max(1, min(16, len(os.sched_getaffinity(0))))
+
+
+def par_oracle(fn, id_i, xi, vi, *rest):
+    """Runs an oracle pair sum (oracle.force / oracle.force_ld) over i-chunks
+    on several threads (the ctypes call releases the GIL) and concatenates
+    the per-i outputs."""
+    from concurrent.futures import ThreadPoolExecutor
+    ni = len(id_i)
+    nt = min(oracle_threads(), max(1, ni))
+    cuts = np.linspace(0, ni, nt + 1).astype(int)
+    with ThreadPoolExecutor(nt) as ex:
+        parts = list(ex.map(lambda k: fn(id_i[cuts[k]:cuts[k + 1]], xi[cuts[k]:cuts[k + 1]],
+                                         vi[cuts[k]:cuts[k + 1]], *rest), range(nt)))
+    return tuple(np.concatenate([p[q] for p in parts]) for q in range(len(parts[0])))

@@ -2,11 +2,12 @@
 GPU library follows the reference ph4 integrator's run on the CPU oracle
 (tests/golden/ph4_c0_energy.npz, made by tests/golden/make_ph4_c0.py).
 
-* FP64 forces: the energy error at every output time up to t = 2 stays
-  within 2x the reference run's (plus 1e-13 for rounding of the energy sum
-  itself), and the block / particle step counts stay within 1 % (the
-  trajectories separate chaotically from 1e-16 force differences, so the
-  counts are not compared exactly beyond the first output);
+* FP64 forces: the energy error at every one of the 21 output times up to
+  t = 10 stays within 2x the reference run's largest error up to that time
+  (plus 1e-13 for rounding of the energy sum itself), and the block /
+  particle step counts stay within
+  1 % (the trajectories separate chaotically from 1e-16 force differences,
+  so the counts are not compared exactly beyond the first output);
 * mixed forces at ph4's default timestep_parameter 0.14
   (src/amuse_ph4/interface.py:514-520): the run reaches t = 2 with step
   counts within 10 % of an FP64 run at the same eta, and an energy error
@@ -26,17 +27,17 @@ from conftest import print_case
 pytestmark = pytest.mark.gpu
 
 GOLD = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "ph4_c0_energy.npz")
-NOUT = 5          # outputs t = 0, 0.5, ..., 2
+NOUT_MIXED = 5    # outputs t = 0, 0.5, ..., 2 for the mixed comparison
 
 
-def run(precision, eta=None, max_blocks=0):
+def run(precision, eta=None, max_blocks=0, nout=None):
     from amuse_amd.hermite import Hermite4
     g = np.load(GOLD)
     m, x, v = ic.plummer(int(g["n"]), int(g["seed"]))
     h = Hermite4(precision=precision)
     h.commit(m, x, v, eps2=float(g["eps2"]), eta=float(g["eta"]) if eta is None else eta)
     E, B, S = [], [], []
-    for t in g["t"][:NOUT]:
+    for t in g["t"][:nout]:
         h.set_precision(precision)
         done = h.evolve(float(t), max_blocks)
         h.set_precision(0)
@@ -51,22 +52,25 @@ def run(precision, eta=None, max_blocks=0):
 
 
 def test_fp64_energy_follows_reference_ph4():
-    print_case("c0 fp64 t=0..2")
+    print_case("c0 fp64 t=0..10")
     g, E, B, S = run(0)
-    Eg = g["E"][:NOUT]
+    assert len(E) == len(g["t"]) == 21
+    Eg = g["E"]
     de = np.abs((E - E[0]) / E[0])
     deg = np.abs((Eg - Eg[0]) / Eg[0])
-    print("t", g["t"][:NOUT], "\n|dE/E| gpu", de, "\n|dE/E| ref", deg,
-          "\nblocks", B, g["blocks"][:NOUT], "\nsteps", S, g["steps"][:NOUT], flush=True)
+    print("t", g["t"], "\n|dE/E| gpu", de, "\n|dE/E| ref", deg,
+          "\nblocks", B, g["blocks"], "\nsteps", S, g["steps"], flush=True)
     assert abs(E[0] - Eg[0]) <= 1e-14 * abs(Eg[0])
-    assert (de <= 2 * deg + 1e-13).all()
-    assert (np.abs(B - g["blocks"][:NOUT]) <= 0.01 * g["blocks"][:NOUT]).all()
-    assert (np.abs(S - g["steps"][:NOUT]) <= 0.01 * g["steps"][:NOUT]).all()
+    # within 2x the reference run's error envelope so far (its error can
+    # pass through zero at one output while ours does at the next)
+    assert (de <= 2 * np.maximum.accumulate(deg) + 1e-13).all()
+    assert (np.abs(B - g["blocks"]) <= 0.01 * g["blocks"]).all()
+    assert (np.abs(S - g["steps"]) <= 0.01 * g["steps"]).all()
 
 
 def test_mixed_energy_error():
     print_case("c0 mixed vs fp64 at eta=0.14, t=0..2")
-    res = {p: run(p, eta=0.14, max_blocks=100000) for p in (0, 1)}
+    res = {p: run(p, eta=0.14, max_blocks=100000, nout=NOUT_MIXED) for p in (0, 1)}
     de = {p: np.abs((r[1] - r[1][0]) / r[1][0]) for p, r in res.items()}
     print("|dE/E| fp64", de[0], "\n|dE/E| mixed", de[1], "\nblocks", res[0][2], res[1][2],
           flush=True)

@@ -1,0 +1,227 @@
+"""Every benchmark configuration at its full size on the GPU.
+
+* c1: N = 65536 Plummer, every j with its own time t_j = t_i - U(0,1)
+  2^-U{3..10}, a and jerk from one FP64 pass; i-blocks of the first 2^k
+  particles (k = 0..16) predicted to t_i, through the ABI
+  (firsthalf + lasthalf2), both paths, against the 80-bit oracle on up to
+  256 sampled i per block: FP64 within 1e-15 sum|term| (jerk 4e-15) and
+  1e-12 relative where the sum does not cancel, nn bit-exact; mixed within
+  2e-6 sum|term|, nn at the minimal r^2 up to FP32 rounding.
+* c4: N = 2^20 Plummer with Salpeter masses on [0.1, 100], M = 1,
+  eps2 = 1e-8: one full-N pass per path (g6x_calc_self) against the oracle
+  on 256 sampled i with the same bars, plus size-independent checks over
+  all 2^20 particles: sum m a = 0 (Newton's third law), every nn is
+  another particle, and the mixed potential energy equals the FP64 one.
+* c3: N = 16384 Plummer, equal masses, eps2 = 1e-4, 256 Hermite steps of
+  1/256 checkpointed every 16, Lagrangian-radius loss: FP32 gradients
+  within 1e-3 of the FP64 ones, FP64 gradients against central finite
+  differences along one direction each for x0, v0 and m.
+* c2: 4096 clusters (N log-uniform in [16, 2048], Kroupa masses, M = 1 per
+  cluster, eps2 = 1e-6), all to t = 1 in mixed precision with autotuned
+  tiles and graph-replayed steps: every cluster reaches t = 1 and the
+  per-cluster |dE/E| (energies from FP64 passes) has median <= 2e-8 and
+  90th percentile <= 2e-7 (measured 8.6e-9 and 8.1e-8).
+(c0 to t = 10 is in test_gpu_c0_energy.py.)
+"""
+import numpy as np
+import pytest
+
+import oracle
+from amuse_amd import ic
+from conftest import par_oracle, print_case
+
+pytestmark = pytest.mark.gpu
+
+COND = {"acc": 1e-15, "jerk": 4e-15, "pot": 1e-15}
+REL = 1e-12
+
+
+def check_plain(out, ex, onn, label):
+    acc, jerk, pot, nn = out
+    ea, ej, ep, sa, sj, sp = ex
+    assert np.array_equal(nn, onn), label
+    for val, e, s, name in ((acc, ea, sa, "acc"), (jerk, ej, sj, "jerk"), (pot, ep, sp, "pot")):
+        d = np.abs(val - e).reshape(len(s), -1)
+        mag = np.abs(e).reshape(len(s), -1)
+        d = np.sqrt((d * d).sum(1))
+        mag = np.sqrt((mag * mag).sum(1))
+        assert (d <= COND[name] * s).all(), (label, name, (d / s).max())
+        ok = mag >= 1e-3 * s
+        if ok.any():
+            assert (d[ok] / mag[ok]).max() <= REL, (label, name, (d[ok] / mag[ok]).max())
+
+
+def check_mixed(out, ex, or2, xi, xp, ids, label):
+    acc, jerk, pot, nn = out
+    ea, ej, ep, sa, sj, sp = ex
+    for val, e, s, name in ((acc, ea, sa, "acc"), (jerk, ej, sj, "jerk"), (pot, ep, sp, "pot")):
+        d = np.abs(val - e).reshape(len(s), -1)
+        d = np.sqrt((d * d).sum(1))
+        assert (d <= 2e-6 * s).all(), (label, name, (d / s).max())
+    order = np.argsort(ids)
+    slot = order[np.searchsorted(ids[order], nn)]
+    assert np.array_equal(ids[slot], nn), label
+    dd = xp[slot] - xi
+    r2 = (dd * dd).sum(1)
+    assert (r2 <= or2 * (1 + 1e-5)).all(), (label, (r2 / or2).max())
+
+
+def test_c1_block_sizes(g6):
+    n, eps2 = 65536, 1e-4
+    rng = np.random.Generator(np.random.PCG64(65536))
+    m, x, v = ic.plummer(n, 1)
+    ids = np.arange(1, n + 1, dtype=np.int32)
+    z = np.zeros((n, 3))
+    g6.set_j_particles(np.arange(n), ids, np.zeros(n), 0.0, m, z, z, z, v, x)
+    g6.set_ti(0.0)
+    g6.calc_self(n, 0, n, eps2)
+    acc, jerk, _, _ = g6.fetch(n)
+    ti = 1.0
+    tj = ti - rng.uniform(0, 1, n) * 2.0 ** -rng.integers(3, 11, n).astype(float)
+    a2, j6 = acc / 2, jerk / 6
+    g6.set_j_particles(np.arange(n), ids, tj, 0.0, m, z, j6, a2, v, x)
+    g6.set_ti(ti)
+    xp, vp = oracle.predict(ti, tj, x, v, a2, j6)
+    samples = {k: (np.arange(1 << k) if k <= 8 else
+                   np.linspace(0, (1 << k) - 1, 256).astype(int)) for k in range(17)}
+    union = np.unique(np.concatenate(list(samples.values())))
+    print_case(f"c1 oracle on {union.size} i")
+    ex = par_oracle(oracle.force_ld, ids[union], xp[union], vp[union], eps2, ids, m, xp, vp)
+    _, _, _, onn, or2 = par_oracle(oracle.force, ids[union], xp[union], vp[union], eps2, ids, m,
+                                   xp, vp)
+    where = {int(i): q for q, i in enumerate(union)}
+    for k in range(17):
+        ni = 1 << k
+        sel = np.arange(ni)
+        pick = np.array([where[int(i)] for i in samples[k]])
+        exk = tuple(e[pick] for e in ex)
+        for prec in (0, 1):
+            print_case(f"c1 ni=2^{k} prec={prec}")
+            g6.set_precision(prec)
+            a, j, p, nn, idx = g6.calc(n, ids[sel], xp[sel], vp[sel], eps2)
+            assert np.array_equal(idx, ids[sel])
+            s = samples[k]
+            out = (a[s], j[s], p[s], nn[s])
+            if prec == 0:
+                check_plain(out, exk, onn[pick], f"c1 k={k}")
+            else:
+                check_mixed(out, exk, or2[pick], xp[s], xp, ids, f"c1 k={k} mixed")
+    g6.set_precision(0)
+
+
+def c4_system(n=1 << 20, seed=20240):
+    rng = np.random.Generator(np.random.PCG64(seed))
+    _, pos, vel = ic.plummer_raw(n, rng)
+    pos, vel = ic.to_nbody_units(pos, vel)
+    m = ic.salpeter(n, rng, 0.1, 100.0, -2.35)
+    m = m / m.sum()
+    pos, vel = ic.center(m, pos, vel)
+    return m, pos, vel
+
+
+def test_c4_full_pass(g6):
+    n, eps2 = 1 << 20, 1e-8
+    m, x, v = c4_system(n)
+    ids = np.arange(1, n + 1, dtype=np.int32)
+    z = np.zeros((n, 3))
+    g6.set_j_particles(np.arange(n), ids, np.zeros(n), 0.0, m, z, z, z, v, x)
+    g6.set_ti(0.0)
+    s = np.linspace(0, n - 1, 256).astype(int)
+    print_case("c4 oracle on 256 i x 2^20 j")
+    ex = par_oracle(oracle.force_ld, ids[s], x[s], v[s], eps2, ids, m, x, v)
+    _, _, _, onn, or2 = par_oracle(oracle.force, ids[s], x[s], v[s], eps2, ids, m, x, v)
+    res = {}
+    for prec in (0, 1):
+        print_case(f"c4 full pass N=2^20 prec={prec}")
+        g6.set_precision(prec)
+        g6.calc_self(n, 0, n, eps2, nn=True)
+        a, j, p, nn = g6.fetch(n)
+        res[prec] = (a, p)
+        out = (a[s], j[s], p[s], nn[s])
+        if prec == 0:
+            check_plain(out, ex, onn, "c4")
+        else:
+            check_mixed(out, ex, or2, x[s], x, ids, "c4 mixed")
+        # over all 2^20 particles
+        assert ((nn >= 1) & (nn <= n) & (nn != ids)).all()
+        ma = (m[:, None] * a).sum(0)
+        scale = (m * np.linalg.norm(a, axis=1)).sum()
+        assert np.linalg.norm(ma) <= (1e-12 if prec == 0 else 1e-5) * scale, prec
+    g6.set_precision(0)
+    U64 = 0.5 * (m * res[0][1]).sum()
+    Umx = 0.5 * (m * res[1][1]).sum()
+    print(f"c4 U fp64 {U64!r} mixed {Umx!r}", flush=True)
+    assert abs(Umx - U64) <= 1e-6 * abs(U64)
+
+
+def test_c3_gradients():
+    torch = pytest.importorskip("torch")
+    from amuse_amd.diff import hermite_integrate, lagrangian_radii, radii_loss
+    n, nsteps, dt, eps2, ck = 16384, 256, 1.0 / 256, 1e-4, 16
+    m, x, v = ic.plummer(n, 3)
+    dev = torch.device("cuda")
+    base = [torch.tensor(q, dtype=torch.float64, device=dev) for q in (x, v, m)]
+    with torch.no_grad():
+        targets = 0.9 * lagrangian_radii(base[0], base[2])
+
+    grads = {}
+    for dtype in (torch.float32, torch.float64):
+        print_case(f"c3 N={n} {nsteps} steps {dtype}")
+        xx, vv, mm = (t.to(dtype).clone().requires_grad_(True) for t in base)
+        xf, _ = hermite_integrate(xx, vv, mm, dt, nsteps, eps2, checkpoint_every=ck)
+        radii_loss(xf, mm, targets.to(dtype)).backward()
+        grads[dtype] = [g.double() for g in (xx.grad, vv.grad, mm.grad)]
+    for name, g32, g64 in zip(("x0", "v0", "m"), grads[torch.float32], grads[torch.float64]):
+        rel = (torch.linalg.vector_norm(g32 - g64) / torch.linalg.vector_norm(g64)).item()
+        print(f"c3 grad {name}: fp32 vs fp64 {rel:.3e}", flush=True)
+        assert rel < 1e-3, (name, rel)
+
+    gen = torch.Generator(device=dev).manual_seed(11)
+
+    def loss_at(p):
+        with torch.no_grad():
+            xf, _ = hermite_integrate(p[0], p[1], p[2], dt, nsteps, eps2, checkpoint_every=0)
+            return float(radii_loss(xf, p[2], targets))
+
+    h = 1e-6
+    for which, name in enumerate(("x0", "v0", "m")):
+        d = torch.randn(base[which].shape, dtype=torch.float64, device=dev, generator=gen)
+        if which == 2:
+            d = d * base[2]          # relative perturbation of the masses
+        plus = [t.clone() for t in base]
+        minus = [t.clone() for t in base]
+        plus[which] += h * d
+        minus[which] -= h * d
+        fd = (loss_at(plus) - loss_at(minus)) / (2 * h)
+        an = float((grads[torch.float64][which] * d).sum())
+        print(f"c3 finite difference {name}: analytic {an!r} fd {fd!r}", flush=True)
+        assert abs(an - fd) <= 1e-6 * abs(fd), (name, an, fd)
+
+
+def test_c2_all_clusters_to_t1():
+    from amuse_amd.ensemble import Ensemble, make_ensemble
+    counts, m, x, v = make_ensemble(4096, 2048)
+    print_case(f"c2: 4096 clusters, {counts.sum()} particles, mixed, t=0..1")
+    ens = Ensemble(counts, m, x, v, eps2=1e-6, dt_param=0.03, precision=1)
+    try:
+        ens.autotune()
+        e0 = ens.energy()
+        ens.start(1.0)
+        rounds = 0
+        while True:
+            r = ens.run(steps_per_graph=16, use_graph=True, max_rounds=8192)
+            rounds += r
+            print(f"  c2 {rounds} step rounds", flush=True)
+            if r < 8192:
+                break
+            assert rounds < 400000
+        e1 = ens.energy()
+        _, _, tc, ns = ens.state()
+    finally:
+        ens.close()
+    assert (tc == 1.0).all(), int((tc != 1.0).sum())
+    de = np.abs((e1 - e0) / e0)
+    med, p90 = float(np.median(de)), float(np.quantile(de, 0.9))
+    print(f"c2 |dE/E| median {med:.3e} p90 {p90:.3e} max {de.max():.3e}, "
+          f"{int(ns.sum())} cluster steps", flush=True)
+    assert med <= 2e-8 and p90 <= 2e-7, (med, p90)
