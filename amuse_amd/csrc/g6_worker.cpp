@@ -20,7 +20,11 @@
 //     among the outputs; input k of call i is in[k * call_count + i]; the
 //     int32 result is output 0 of the ints, so int outputs start at 1;
 //   * an unknown id answers with the error flag (flags word 256) and one
-//     string.
+//     string;
+//   * a header is checked before anything is allocated: a call count above
+//     kMaxCallCount, a negative length, or arrays that would hold more than
+//     kMaxMessageBytes in all, end the session (the stream cannot be
+//     re-synchronised), as a closed channel does.
 //
 // Functions: ph4's interface (src/amuse_ph4/interface.cc, interface.py,
 // src/amuse/community/interface/gd.py) with the same names, parameters and
@@ -75,6 +79,11 @@ int function_id(const char *name)
 enum { H_FLAGS, H_CALL_ID, H_FUNCTION_ID, H_CALL_COUNT, H_INTS, H_LONGS, H_FLOATS, H_DOUBLES,
        H_BOOLS, H_STRINGS, H_UNITS, H_SIZE };
 constexpr int kErrorFlag = 256;
+// Limits on what one message may declare (AMUSE splits long vectorised calls
+// into messages of at most 1,000,000 calls, src/amuse/rfi/channel.py
+// max_message_length; ph4's new_particle for 2^20 particles is ~80 MB).
+constexpr int32_t kMaxCallCount = 1 << 24;
+constexpr size_t kMaxMessageBytes = size_t(1) << 30;
 
 struct Message {
     int32_t header[H_SIZE] = {};
@@ -119,6 +128,23 @@ bool recv_vec(int fd, std::vector<T> &v, int n)
     return n == 0 || recv_all(fd, v.data(), sizeof(T) * size_t(n));
 }
 
+// Bytes the header's array lengths declare (strings without their
+// characters, which are bounded once their lengths are read), or SIZE_MAX
+// when a length is negative or the call count is out of range.
+size_t declared_bytes(const int32_t *h)
+{
+    if (h[H_CALL_COUNT] < 0 || h[H_CALL_COUNT] > kMaxCallCount) return SIZE_MAX;
+    const int32_t n[] = {h[H_INTS], h[H_LONGS], h[H_FLOATS], h[H_DOUBLES], h[H_BOOLS],
+                         h[H_STRINGS], h[H_UNITS]};
+    const size_t w[] = {4, 8, 4, 8, 1, 4, 8};
+    size_t total = 0;
+    for (int k = 0; k < 7; k++) {
+        if (n[k] < 0) return SIZE_MAX;
+        total += w[k] * size_t(n[k]);
+    }
+    return total;
+}
+
 template <class T>
 bool send_vec(int fd, const std::vector<T> &v)
 {
@@ -133,6 +159,11 @@ bool receive(int fd, Message &m)
         fprintf(stderr, "g6_worker: message endianness does not match\n");
         return false;
     }
+    size_t budget = declared_bytes(m.header);
+    if (budget > kMaxMessageBytes) {
+        fprintf(stderr, "g6_worker: message header declares invalid or oversized arrays\n");
+        return false;
+    }
     if (!recv_vec(fd, m.ints, m.header[H_INTS]) || !recv_vec(fd, m.longs, m.header[H_LONGS]) ||
         !recv_vec(fd, m.floats, m.header[H_FLOATS]) ||
         !recv_vec(fd, m.doubles, m.header[H_DOUBLES]) ||
@@ -145,6 +176,10 @@ bool receive(int fd, Message &m)
     for (int32_t l : len) {
         if (l < 0) return false;
         total += size_t(l) + 1;
+    }
+    if (total > kMaxMessageBytes - budget) {
+        fprintf(stderr, "g6_worker: message strings exceed the size limit\n");
+        return false;
     }
     std::vector<char> chars(total);
     if (total && !recv_all(fd, chars.data(), total)) return false;

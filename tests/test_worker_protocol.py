@@ -13,6 +13,7 @@ the UTF-8 characters, each string followed by a NUL); ph4's id assignment
 computes its ids with its own CRC-32; every call below succeeding shows the
 two sides agree.
 """
+import socket
 import struct
 import zlib
 
@@ -150,3 +151,82 @@ def test_error_reply_raises(worker, monkeypatch):
     with pytest.raises(WorkerError, match="unknown function id"):
         worker.call("no_such_function")
     assert worker.call("get_number_of_particles")[1]["number_of_particles"] == 0
+
+
+def _raw_session():
+    """A worker whose socket the test writes raw bytes to."""
+    w = Ph4Worker()
+    assert w.call("initialize_code") == (0, {})
+    return w
+
+
+def _expect_session_end(w):
+    """The worker closes the channel without replying and exits with 0."""
+    w.sock.settimeout(60)
+    assert w.sock.recv(1) == b""
+    w.sock.close()
+    w.sock = None
+    assert w.process.wait(timeout=60) == 0
+
+
+def _header(fid, call_count, counts, flags=(0, 0, 0, 0), call_id=1):
+    return struct.pack("<4b10i", *flags, call_id, fid, call_count, *counts)
+
+
+@pytest.mark.parametrize("counts, call_count", [
+    ((0, 0, 0, -1, 0, 0, 0), 1),                  # negative doubles
+    ((-5, 0, 0, 0, 0, 0, 0), 1),                  # negative ints
+    ((0, 0, 0, 0, 0, 0, -1), 1),                  # negative units
+    ((0, 0, 0, 2 ** 31 - 1, 0, 0, 0), 1),         # 16 GiB of doubles
+    ((2 ** 31 - 1,) * 7, 1),                      # every array at its maximum
+    ((0, 0, 0, 0, 0, 0, 0), 2 ** 31 - 1),         # outputs for 2^31 calls
+    ((0, 0, 0, 0, 0, 0, 0), -3),                  # negative call count
+])
+def test_malformed_header_ends_session(counts, call_count):
+    w = _raw_session()
+    w.sock.sendall(_header(function_id("get_time"), call_count, counts))
+    _expect_session_end(w)
+
+
+@pytest.mark.parametrize("lengths", [[-1], [2 ** 31 - 1], [2 ** 30, 2 ** 30]])
+def test_malformed_string_lengths_end_session(lengths):
+    w = _raw_session()
+    w.sock.sendall(_header(function_id("get_time"), 1, (0, 0, 0, 0, 0, len(lengths), 0))
+                   + struct.pack(f"<{len(lengths)}i", *lengths))
+    _expect_session_end(w)
+
+
+def test_big_endian_and_truncated_messages_end_session():
+    w = _raw_session()
+    w.sock.sendall(_header(function_id("get_time"), 1, (0,) * 7, flags=(1, 0, 0, 0)))
+    _expect_session_end(w)
+    # header promises 3 doubles, the channel closes after one
+    w = _raw_session()
+    w.sock.sendall(_header(function_id("set_eps2"), 1, (0, 0, 0, 3, 0, 0, 0))
+                   + struct.pack("<d", 1.0))
+    w.sock.shutdown(socket.SHUT_WR)
+    _expect_session_end(w)
+    # a header cut short
+    w = _raw_session()
+    w.sock.sendall(_header(function_id("get_time"), 1, (0,) * 7)[:17])
+    w.sock.shutdown(socket.SHUT_WR)
+    _expect_session_end(w)
+
+
+def test_inconsistent_counts_answer_with_error():
+    """Counts that are valid but do not fit the function get an error reply
+    and the session goes on."""
+    w = _raw_session()
+    try:
+        # set_eps2 takes one double per call: 2 calls with 3 doubles
+        w.sock.sendall(_header(function_id("set_eps2"), 2, (0, 0, 0, 3, 0, 0, 0))
+                       + struct.pack("<3d", 1.0, 2.0, 3.0))
+        rep = Message.read(w._recv)
+        assert rep.error and "set_eps2" in rep.strings[0]
+        # call count 0
+        w.sock.sendall(_header(function_id("get_time"), 0, (0,) * 7))
+        rep = Message.read(w._recv)
+        assert rep.error
+        assert w.call("get_eps2")[0] == 0
+    finally:
+        assert w.stop() == 0
