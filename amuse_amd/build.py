@@ -2,6 +2,11 @@
 
     python -m amuse_amd.build          # libg6.so, libhermite4.so, g6_worker
     python -m amuse_amd.build --isa    # also dump the gfx950 assembly to _build/
+    python -m amuse_amd.build --install PREFIX
+        # copies the GRAPE-6 library where AMUSE's build looks for the reference
+        # ones: PREFIX/lib/libg6.so (g6lib's name) and libsapporo.so
+        # (sapporo_light's name, a link to the same file), PREFIX/include/g6lib.h
+        # and g6x.h, PREFIX/lib/pkgconfig/g6lib.pc and sapporo_light.pc
 
 The libraries land in amuse_amd/lib/ so they travel with the source tree; no
 JIT cache and no site-packages install is involved.
@@ -99,11 +104,64 @@ def build_exe(name: str) -> str:
     return out
 
 
+# pkg-config modules as lib/g6lib/Makefile:12-27 and lib/sapporo_light/Makefile:12-27
+# write them at install time; AMUSE's configure probes g6_npipes in -lg6 and
+# get_device_count in -lsapporo first and falls back to these modules
+# (support/shared/m4/amuse_lib.m4:17-45,126-136).
+PKG_MODULES = {
+    "g6lib": ("GRAPE-6 ABI on AMD Instinct MI355X (HIP, gfx950)", "g6"),
+    "sapporo_light": ("GRAPE-6 ABI on AMD Instinct MI355X, under sapporo_light's name", "sapporo"),
+}
+HEADERS = ["g6lib.h", "g6x.h"]
+
+
+def install(prefix: str) -> list[str]:
+    """Installs the built libg6.so under the names the reference libraries
+    use.  libsapporo.so is a symbolic link: the library has no SONAME, so a
+    program linked with -lsapporo records libsapporo.so and finds it through
+    the link."""
+    src = os.path.join(LIBDIR, "libg6.so")
+    if not os.path.exists(src):
+        raise RuntimeError(f"{src} is not built; run amuse_amd.build")
+    prefix = os.path.abspath(prefix)
+    lib, inc, pc = (os.path.join(prefix, d) for d in ("lib", "include", "lib/pkgconfig"))
+    for d in (lib, inc, pc):
+        os.makedirs(d, exist_ok=True)
+    out = []
+    dst = os.path.join(lib, "libg6.so")
+    shutil.copy2(src, dst + ".tmp")
+    os.replace(dst + ".tmp", dst)
+    out.append(dst)
+    link = os.path.join(lib, "libsapporo.so")
+    if os.path.lexists(link):
+        os.remove(link)
+    os.symlink("libg6.so", link)
+    out.append(link)
+    for h in HEADERS:
+        shutil.copy2(os.path.join(ROOT, "include", h), os.path.join(inc, h))
+        out.append(os.path.join(inc, h))
+    for name, (desc, libname) in PKG_MODULES.items():
+        path = os.path.join(pc, name + ".pc")
+        with open(path, "w") as f:
+            f.write(f"prefix={prefix}\nincludedir=${{prefix}}/include\nlibdir=${{prefix}}/lib\n\n"
+                    f"Name: {name}\nDescription: {desc}\nVersion: 1.0\n"
+                    f"Cflags: -I${{includedir}}\nLibs: -L${{libdir}} -l{libname}\n")
+        out.append(path)
+    return out
+
+
 def build_all(isa: bool = False) -> list[str]:
     return ([build_lib(n, isa) for n in LIBS] + [build_cpu_lib(n) for n in CPU_LIBS]
             + [build_exe(n) for n in EXES])
 
 
 if __name__ == "__main__":
-    for p in build_all(isa="--isa" in sys.argv):
-        print("built", p)
+    if "--install" in sys.argv:
+        at = sys.argv.index("--install")
+        if at + 1 >= len(sys.argv):
+            sys.exit("usage: python -m amuse_amd.build --install PREFIX")
+        for p in install(sys.argv[at + 1]):
+            print("installed", p)
+    else:
+        for p in build_all(isa="--isa" in sys.argv):
+            print("built", p)
