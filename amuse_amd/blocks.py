@@ -31,6 +31,7 @@ def _bind(lib):
         "g6d_steps": (C.c_int, [C.c_void_p, _D, _D]),
         "g6d_counters": (C.c_int, [C.c_void_p, _LL, _LL, _D]),
         "g6d_set_precision": (C.c_int, [C.c_void_p, C.c_int]),
+        "g6d_synchronize": (C.c_int, [C.c_void_p]),
         "g6d_last_error": (C.c_char_p, [C.c_void_p]),
         "g6d_free": (None, [C.c_void_p]),
     }
@@ -92,6 +93,11 @@ class DeviceHermite4:
     def steps(self, nblocks: int) -> None:
         self._check(self.lib.g6d_run(self.h, float("inf"), int(nblocks), self.steps_per_graph,
                                      None), "g6d_run")
+
+    def synchronize(self) -> None:
+        """Every particle behind the system time steps to it (ph4's
+        synchronize_all)."""
+        self._check(self.lib.g6d_synchronize(self.h), "g6d_synchronize")
 
     def energy(self):
         k, u = C.c_double(), C.c_double()

@@ -30,7 +30,7 @@ DEVICE_FLAGS = ["-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops"]
 
 LIBS = {
     "libg6.so": ["g6_kernels.hip", "g6_abi.hip", "g6_clusters.hip", "g6_adjoint.hip",
-                 "g6_blocks.hip"],
+                 "g6_blocks.hip", "g6_spatial.hip"],
 }
 # host-only C++ (no device code): built with g++
 CPU_LIBS = {

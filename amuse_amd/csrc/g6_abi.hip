@@ -22,6 +22,7 @@
 #include <cstdlib>
 #include <cstring>
 #include <string>
+#include <unordered_map>
 #include <vector>
 
 #define G6_API extern "C" __attribute__((visibility("default")))
@@ -60,6 +61,27 @@ struct Unit {
 
     std::vector<JRecord> stage;
     std::vector<int> stage_pos;  // slot -> index in stage, -1 if none
+    // Ids of the slots as the set_j_particle calls so far leave them (-1:
+    // empty) and how many slots hold each id.  g6x_calc_self leaves out the
+    // per-pair id compare in work items whose slots cannot meet only while
+    // every slot below nj holds an id and no id is held twice.
+    std::vector<int> slot_id;
+    std::unordered_map<int, int> id_count;
+    long long id_repeats = 0;  // sum over ids of (slots holding it - 1)
+    int n_filled = 0;          // slots holding an id other than -1
+
+    // Full passes of the mixed path in spatial order (g6x_calc_self over
+    // every slot): the sorted copy of the predicted store, the permutation,
+    // the tile boxes, result rows in sorted order, the sort's buffers.
+    bool spatial = true;
+    JStore sj;
+    int sp_cap = 0;
+    int *sp_perm = nullptr;
+    float *sp_tbox = nullptr;
+    double *sp_rows = nullptr;  // [8][ib.cap]
+    int sp_rows_cap = 0;
+    void *sp_base = nullptr;
+    g6k::SpatialBuf sp_buf;
     JRecord *h_stage = nullptr;
     JRecord *d_hstage = nullptr;  // the device's address of h_stage
     size_t h_stage_cap = 0;
@@ -238,7 +260,36 @@ int grow_jstore(int need)
     }
     U.js = ns;
     U.stage_pos.resize(cap, -1);
+    U.slot_id.resize(cap, -1);
     return 0;
+}
+
+// All slots empty (the device store is reset alongside).
+void clear_ids()
+{
+    std::fill(U.slot_id.begin(), U.slot_id.end(), -1);
+    U.id_count.clear();
+    U.id_repeats = 0;
+    U.n_filled = 0;
+}
+
+void note_id(int address, int id)
+{
+    const int old = U.slot_id[address];
+    if (old == id) return;
+    if (old != -1) {
+        auto it = U.id_count.find(old);
+        if (it->second > 1) U.id_repeats--;
+        if (--it->second == 0) U.id_count.erase(it);
+        U.n_filled--;
+    }
+    if (id != -1) {
+        int &c = U.id_count[id];
+        if (c > 0) U.id_repeats++;
+        c++;
+        U.n_filled++;
+    }
+    U.slot_id[address] = id;
 }
 
 int grow_iblock(int need)
@@ -370,6 +421,8 @@ int auto_chunks(int iblocks, int ni, int nj)
     return nchunk;
 }
 
+ForceCfg with_chunks(ForceCfg c, int nchunk, int ni, int nj, long long max_records);
+
 ForceCfg choose_cfg(int ni, int nj)
 {
     ForceCfg c;
@@ -426,19 +479,25 @@ ForceCfg choose_cfg(int ni, int nj)
     }
     const int ib = (g6k::kWG / njs) * ipl;
     const int iblocks = (ni + ib - 1) / ib;
-    const int max_chunks = std::max(1, (nj + g6k::kTJ - 1) / g6k::kTJ);
     if (nchunk <= 0) nchunk = auto_chunks(iblocks, ni, nj);
-    nchunk = std::min(nchunk, max_chunks);
-    while (nchunk > 1 && (long long)nchunk * ni > kPartBudget) nchunk /= 2;
+    c.njs = njs;
+    c.ipl = ipl;
+    c.smem = smem != 0 && njs == 1;
+    return with_chunks(c, nchunk, ni, nj, kPartBudget);
+}
+
+// cfg with at most `nchunk` j-chunks (whole tiles each) and at most
+// `max_records` (chunk, i) partial records.
+ForceCfg with_chunks(ForceCfg c, int nchunk, int ni, int nj, long long max_records)
+{
+    const int max_chunks = std::max(1, (nj + g6k::kTJ - 1) / g6k::kTJ);
+    nchunk = std::max(1, std::min(nchunk, max_chunks));
+    while (nchunk > 1 && (long long)nchunk * ni > max_records) nchunk /= 2;
     int jchunk = (nj + nchunk - 1) / nchunk;
     jchunk = ((jchunk + g6k::kTJ - 1) / g6k::kTJ) * g6k::kTJ;
     if (jchunk <= 0) jchunk = g6k::kTJ;
-    nchunk = std::max(1, (nj + jchunk - 1) / jchunk);
-    c.njs = njs;
-    c.ipl = ipl;
-    c.nchunk = nchunk;
+    c.nchunk = std::max(1, (nj + jchunk - 1) / jchunk);
     c.jchunk = jchunk;
-    c.smem = smem != 0 && njs == 1;
     return c;
 }
 
@@ -474,7 +533,8 @@ IBlock io_view(int ni)
     return b;
 }
 
-int run_force(int ni, int nj, double eps2, bool want_nn)
+// self_i0: as g6k::launch_force (>= 0 only for g6x_calc_self).
+int run_force(int ni, int nj, double eps2, bool want_nn, int self_i0 = -1)
 {
     if (ni <= 0) { U.last_ni = 0; return 0; }
     ForceCfg cfg = choose_cfg(ni, std::max(nj, 1));
@@ -494,7 +554,7 @@ int run_force(int ni, int nj, double eps2, bool want_nn)
     Unit::Ev ev;
     timed_begin(ev, 0, double(ni) * double(nj));
     HIPCHK(g6k::launch_force(U.js, b, ni, nj, eps2, U.prec, want_nn, cfg, U.stream, pred, U.l_ti,
-                             U.compat),
+                             U.compat, self_i0),
            "force kernel");
     timed_end(ev);
     if (cfg.nchunk > 1) {
@@ -533,6 +593,14 @@ int do_open(int cluster)
         if (U.ib.base) (void)hipFree(U.ib.base);
         if (U.ib.part_base) (void)hipFree(U.ib.part_base);
         if (U.d_stage) (void)hipFree(U.d_stage);
+        if (U.sp_base) (void)hipFree(U.sp_base);
+        if (U.sp_rows) (void)hipFree(U.sp_rows);
+        if (U.sp_buf.base) (void)hipFree(U.sp_buf.base);
+        if (U.sp_buf.tmp) (void)hipFree(U.sp_buf.tmp);
+        U.sp_base = nullptr;
+        U.sp_rows = nullptr;
+        U.sp_cap = U.sp_rows_cap = 0;
+        U.sp_buf = g6k::SpatialBuf();
         (void)hipStreamDestroy(U.stream);
         U.js = JStore();
         U.ib = IBlock();
@@ -562,6 +630,7 @@ int do_open(int cluster)
     HIPCHK(hipStreamSynchronize(U.stream), "open sync");
     U.stage.clear();
     std::fill(U.stage_pos.begin(), U.stage_pos.end(), -1);
+    clear_ids();
     U.nj_hw = 0;
     U.pred_valid = false;
     U.k18_nonzero = false;
@@ -596,6 +665,7 @@ int do_set_j(int address, int index, double tj, double dtj, double mass, const d
     int &pos = U.stage_pos[address];
     if (pos >= 0) U.stage[pos] = r;
     else { pos = int(U.stage.size()); U.stage.push_back(r); }
+    note_id(address, index);
     if (address + 1 > U.nj_hw) U.nj_hw = address + 1;
     return 0;
 }
@@ -771,6 +841,7 @@ G6_API int g6_reset_(int *)
     if (rc) return rc;
     U.stage.clear();
     std::fill(U.stage_pos.begin(), U.stage_pos.end(), -1);
+    clear_ids();
     HIPCHK(g6k::launch_reset(U.js, 0, U.js.cap, U.stream), "reset");
     HIPCHK(hipStreamSynchronize(U.stream), "reset sync");
     U.nj_hw = 0;
@@ -944,6 +1015,96 @@ G6_API int g6x_set_npipes(int npipes)
 }
 G6_API const char *g6x_last_error(void) { return U.err.c_str(); }
 
+// Full passes at least this long take the spatial order (mixed path).
+constexpr int kSpatialMin = 1 << 17;
+
+int ensure_spatial(int nj)
+{
+    if (nj > U.sp_cap) {
+        HIPCHK(hipStreamSynchronize(U.stream), "spatial sync");
+        if (U.sp_base) (void)hipFree(U.sp_base);
+        U.sp_base = nullptr;
+        U.sp_cap = 0;
+        int cap = 1 << 16;
+        while (cap < nj) cap *= 2;
+        const size_t js = g6k::jstore_bytes(cap);
+        const size_t tiles = size_t(cap / g6k::kTJ + 1);
+        void *b = nullptr;
+        hipError_t e = hipMalloc(&b, js + sizeof(int) * size_t(cap) + sizeof(float) * 6 * tiles + 512);
+        if (e != hipSuccess) return fail(G6X_ERR_NOMEM, "hipMalloc spatial store", e);
+        U.sp_base = b;
+        g6k::jstore_carve(U.sj, b, cap);
+        char *q = static_cast<char *>(b) + ((js + 255) & ~size_t(255));
+        U.sp_perm = reinterpret_cast<int *>(q);
+        q += (sizeof(int) * size_t(cap) + 255) & ~size_t(255);
+        U.sp_tbox = reinterpret_cast<float *>(q);
+        U.sp_cap = cap;
+    }
+    if (U.ib.cap > U.sp_rows_cap) {
+        HIPCHK(hipStreamSynchronize(U.stream), "spatial sync");
+        if (U.sp_rows) (void)hipFree(U.sp_rows);
+        U.sp_rows = nullptr;
+        U.sp_rows_cap = 0;
+        hipError_t e = hipMalloc(reinterpret_cast<void **>(&U.sp_rows),
+                                 sizeof(double) * 8 * size_t(U.ib.cap));
+        if (e != hipSuccess) return fail(G6X_ERR_NOMEM, "hipMalloc spatial rows", e);
+        U.sp_rows_cap = U.ib.cap;
+    }
+    return 0;
+}
+
+// Full pass (i = every slot) of the mixed path in spatial order: the slots
+// sorted along a Morton curve (g6k::spatial_order), the i-block taken from
+// the sorted copy, the force kernel for spatially ordered slots, the fold,
+// and the result rows put back in slot order.  False with nothing done when
+// the shape has no spatial kernel.
+int run_force_spatial(int nj, double eps2, bool want_nn, bool &done)
+{
+    done = false;
+    const ForceCfg cfg = choose_cfg(nj, nj);
+    if (!g6k::spatial_cfg_supported(cfg)) return 0;
+    int rc = ensure_spatial(nj);
+    if (rc) return rc;
+    rc = ensure_partials(cfg.nchunk, nj);
+    if (rc) return rc;
+    Unit::Ev ev0;
+    timed_begin(ev0, 2, 0.0);
+    HIPCHK(g6k::spatial_order(U.js, nj, U.sj, U.sp_perm, U.sp_tbox, U.sp_buf, U.stream),
+           "spatial order");
+    HIPCHK(g6k::launch_iblock_self(U.sj, U.ib, 0, nj, g6k::kMixed, U.stream), "iblock sorted");
+    timed_end(ev0);
+    IBlock b = U.ib;
+    b.out = U.sp_rows;
+    b.nn = reinterpret_cast<int *>(U.sp_rows + 7 * size_t(U.ib.cap));
+    Unit::Ev ev;
+    timed_begin(ev, 0, double(nj) * double(nj));
+    HIPCHK(g6k::launch_force(U.sj, b, nj, nj, eps2, g6k::kMixed, want_nn, cfg, U.stream, false,
+                             0.0, U.compat, -1, U.sp_tbox),
+           "force kernel (spatial)");
+    timed_end(ev);
+    Unit::Ev ev2;
+    timed_begin(ev2, 1, 0.0);
+    if (cfg.nchunk > 1)
+        HIPCHK(g6k::launch_reduce(U.sj, b, nj, cfg.nchunk, g6k::kMixed, want_nn, U.stream),
+               "reduce kernel");
+    HIPCHK(g6k::launch_unsort_rows(b.out, b.nn, U.ib.cap, U.sp_perm, nj, U.ib.out, U.ib.nn,
+                                   U.ib.cap, want_nn, U.stream),
+           "unsort rows");
+    timed_end(ev2);
+    U.out_direct = false;
+    U.launched = true;
+    U.launched_nn = want_nn;
+    U.last_ni = nj;
+    done = true;
+    return 0;
+}
+
+G6_API int g6x_set_spatial(int on)
+{
+    U.spatial = on != 0;
+    return 0;
+}
+
 G6_API int g6x_calc_self(int nj, int i0, int ni, double eps2, int want_nn)
 {
     int rc = check_open();
@@ -963,10 +1124,17 @@ G6_API int g6x_calc_self(int nj, int i0, int ni, double eps2, int want_nn)
     if (rc) return rc;
     rc = ensure_predicted(U.ti);
     if (rc) return rc;
-    HIPCHK(g6k::launch_iblock_self(U.js, U.ib, i0, ni, U.prec, U.stream), "iblock self");
     U.latched = false;
     U.in_direct = false;  // the i-block is in the device rows
-    return run_force(ni, nj, eps2, want_nn != 0);
+    const bool distinct = U.id_repeats == 0 && U.n_filled == nj && U.nj_hw == nj;
+    if (U.spatial && distinct && U.prec == g6k::kMixed && i0 == 0 && ni == nj &&
+        nj >= kSpatialMin) {
+        bool done = false;
+        rc = run_force_spatial(nj, eps2, want_nn != 0, done);
+        if (rc || done) return rc;
+    }
+    HIPCHK(g6k::launch_iblock_self(U.js, U.ib, i0, ni, U.prec, U.stream), "iblock self");
+    return run_force(ni, nj, eps2, want_nn != 0, distinct ? i0 : -1);
 }
 
 G6_API int g6x_fetch(int ni, double acc[][3], double jerk[][3], double pot[], int nn[])

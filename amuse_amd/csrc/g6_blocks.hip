@@ -130,20 +130,25 @@ __global__ void __launch_bounds__(256) kb_first_min(const double *__restrict__ t
 // into the i-block rows; every j-slot is predicted to it as well (the
 // library's predictor at set_ti); the other particles' t + dt, unchanged by
 // this step, go into the next step's minimum tmin[par ^ 1].
+//
+// sync = 1 (g6d_synchronize): ph4's synchronize_all (jdata.cc:874-900 with
+// idata::set_list_sync, idata.cc:773-785) -- the block is every particle
+// behind the system time, stepped to it; t_end and the budget do not apply.
 __global__ void __launch_bounds__(256) kb_select(State s, JStore js, IBlock b, StepCtl *c,
-                                                 int prec, int compat, int par)
+                                                 int prec, int compat, int par, int sync)
 {
     if (c->done) return;
     const bool first = blockIdx.x == 0 && threadIdx.x == 0;
-    if (!(c->system_time < c->t_end)) {
+    if (!sync && !(c->system_time < c->t_end)) {
         if (first) c->done = 1;  // t_end reached
         return;
     }
-    if (c->budget == 0) {
+    if (!sync && c->budget == 0) {
         if (first) c->done = 2;  // block budget spent
         return;
     }
-    const double tnext = __longlong_as_double(static_cast<long long>(c->tmin[par]));
+    const double tnext = sync ? c->system_time
+                              : __longlong_as_double(static_cast<long long>(c->tmin[par]));
     if (first) c->tnext = tnext;
     double lo = __builtin_inf();
     for (int j = blockIdx.x * blockDim.x + threadIdx.x; j < s.n; j += gridDim.x * blockDim.x) {
@@ -159,7 +164,7 @@ __global__ void __launch_bounds__(256) kb_select(State s, JStore js, IBlock b, S
             g6k::write_pred(js, j, prec, js.id[j], pr);
         }
         const double tj = s.t[j] + s.dt[j];
-        if (tj != tnext) {
+        if (sync ? !(s.t[j] < tnext) : tj != tnext) {
             lo = fmin(lo, tj);
             continue;
         }
@@ -370,12 +375,12 @@ __global__ void kb_arm(StepCtl *c, double t_end, long long budget)
 // ---------------------------------------------------------------------------
 constexpr int kGrid = 1024;  // workgroups of the per-particle kernels
 
-int enqueue_step(Driver *D, int par)
+int enqueue_step(Driver *D, int par, int sync = 0)
 {
     const int n = D->st.n;
     const int g = std::min(nblk(n, 256), kGrid);
     hipLaunchKernelGGL(kb_select, dim3(g), dim3(256), 0, D->stream, D->st, D->js, D->ib, D->ctl,
-                       D->prec, D->compat, par);
+                       D->prec, D->compat, par, sync);
     DCK(g6k::launch_step_force(D->js, D->ib, D->part, n, D->eps2, D->prec, D->ctl, par, par,
                                false, D->stream));
     // enough waves for one block row each when the partials are folded per
@@ -428,9 +433,11 @@ void release(Driver *D)
 int setup(Driver *D, int n, const int *ids, const double *m, const double *x, const double *v)
 {
     DCK(hipStreamCreateWithFlags(&D->stream, hipStreamNonBlocking));
-    // state: id, list (ints) and m pot t dt x3 v3 a3 j3 (doubles)
+    // state: id, list (ints) and m pot t dt x3 v3 a3 j3 (doubles), each
+    // array starting on a 256-byte boundary (up to 255 bytes of padding
+    // after each of the 18)
     const size_t nd = size_t(n);
-    const size_t bytes = nd * sizeof(double) * 16 + nd * sizeof(int) * 2 + 4096;
+    const size_t bytes = nd * sizeof(double) * 16 + nd * sizeof(int) * 2 + 19 * 256;
     DCK(hipMalloc(&D->state_base, bytes));
     char *p = static_cast<char *>(D->state_base);
     auto take = [&](size_t b) {
@@ -626,6 +633,23 @@ G6D_API long long g6d_run(void *h, double t_end, long long max_blocks, int steps
     }
     if (reached) *reached = D->h_ctl->done == 1;
     return D->h_ctl->blocks - b0;
+}
+
+// Every particle behind the system time takes one step to it (ph4's
+// synchronize_all, src/amuse_ph4/src/jdata.cc:874-900): a block step whose
+// block is that set (counted as a block step, as ph4 counts it), with the
+// corrector and the next step as in any block step.  Afterwards every
+// particle's time is the system time.
+G6D_API int g6d_synchronize(void *h)
+{
+    if (!h) return G6X_ERR_ARG;
+    Driver *D = static_cast<Driver *>(h);
+    // the pending block time to slot 0 (parity 0 below), no end time
+    hipLaunchKernelGGL(g6b::kb_arm, dim3(1), dim3(1), 0, D->stream, D->ctl,
+                       __builtin_inf(), -1ll);
+    if (g6b::enqueue_step(D, 0, 1)) return G6X_ERR_HIP;
+    if (g6b::read_ctl(D)) return G6X_ERR_HIP;
+    return 0;
 }
 
 // Kinetic and potential energy at the system time (a full-N FP64 pass with

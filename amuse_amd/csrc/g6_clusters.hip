@@ -28,6 +28,7 @@
 // The four kernels of a step contain no memset and no host synchronisation,
 // so a sequence of steps is captured once into a hipGraph and replayed.
 #include "g6_kernels.h"
+#include <hip/hip_cooperative_groups.h>
 #include "../../include/g6x.h"
 
 #include <algorithm>
@@ -67,7 +68,25 @@ struct Dev {
     int nlist = 0;
     int *queue = nullptr, *nactive = nullptr;
     double t_end = 1.0, dt_param = 0.03, eps2 = 1e-6, dt_limit = 0.03125, dt_floor = 0x1p-40;
+    // j per LDS tile (<= kTJ) and lanes per i of each size class
+    int tj_class[kClasses] = {kTJ, kTJ, kTJ, kTJ, kTJ, kTJ, kTJ, kTJ};
+    int njs_class[kClasses] = {16, 16, 4, 4, 1, 1, 1, 1};
+    // work list of the persistent run (kc_run), built on the device: items
+    // (cluster, first row, lanes per row) of the active clusters, clusters
+    // taken in the order corder (largest first); ctl = {items, clusters
+    // covered, boost, built}
+    int *wl_c = nullptr, *wl_i0 = nullptr, *wl_njs = nullptr, *corder = nullptr, *ctl = nullptr;
+    int wl_cap = 0;
+    int fill_target = 512;  // items a rebuilt list aims at (see build_items)
 };
+
+// size class of a cluster of n particles: <=16, <=32, ..., <=2048, more
+__host__ __device__ inline int size_class_d(int n)
+{
+    int k = 0, lim = 16;
+    while (n > lim && k < kClasses - 1) { lim *= 2; k++; }
+    return k;
+}
 
 __device__ __forceinline__ unsigned long long dbits(double x)
 {
@@ -77,15 +96,8 @@ __device__ __forceinline__ unsigned long long dbits(double x)
 // ---------------------------------------------------------------------------
 // predictor (FP64), also resets the work queue of the force kernel
 // ---------------------------------------------------------------------------
-__global__ void __launch_bounds__(256) kc_predict(Dev d, int zero_step)
+__device__ __forceinline__ void predict_particle(const Dev &d, int p, double dt)
 {
-    const int q = blockIdx.x * blockDim.x + threadIdx.x;
-    if (q == 0) *d.queue = 0;
-    if (q >= d.nlist) return;
-    const int p = d.plist[q];
-    const int c = d.pc[p];
-    if (!d.active[c]) return;
-    const double dt = zero_step ? 0.0 : d.dt[c];
 #pragma unroll
     for (int k = 0; k < 3; k++) {
         const double x = d.x[k][p], v = d.v[k][p], a = d.a[k][p], j = d.j[k][p];
@@ -99,6 +111,17 @@ __global__ void __launch_bounds__(256) kc_predict(Dev d, int zero_step)
         d.lx[k][p] = float(xp - double(hi));
         d.fv[k][p] = float(vp);
     }
+}
+
+__global__ void __launch_bounds__(256) kc_predict(Dev d, int zero_step)
+{
+    const int q = blockIdx.x * blockDim.x + threadIdx.x;
+    if (q == 0) *d.queue = 0;
+    if (q >= d.nlist) return;
+    const int p = d.plist[q];
+    const int c = d.pc[p];
+    if (!d.active[c]) return;
+    predict_particle(d, p, zero_step ? 0.0 : d.dt[c]);
 }
 
 // ---------------------------------------------------------------------------
@@ -129,28 +152,23 @@ __device__ __forceinline__ void two_sum_d(double &hi, double &lo, double x)
     hi = s;
 }
 
+union TileU {
+    TileF f;
+    TileD dd;
+};
+
+// Work item (cluster c, rows i0 .. i0 + kWG/njs - 1 of it, njs lanes per
+// row) of one force pass; every thread of the workgroup takes part (tile
+// loads and barriers), so the caller's control flow is workgroup-uniform.
 template <bool MIXED>
-__global__ void __launch_bounds__(kWG) kc_force(Dev d)
+__device__ __forceinline__ void force_item(const Dev &d, TileU &tile, int c, int i0, int njs)
 {
-    __shared__ union {
-        TileF f;
-        TileD dd;
-    } tile;
-    __shared__ int s_item;
     const int t = threadIdx.x;
-    for (;;) {
-        if (t == 0) s_item = atomicAdd(d.queue, 1);
-        __syncthreads();
-        const int it = s_item;
-        __syncthreads();
-        if (it >= d.nitems) break;
-        const int c = d.item_c[it];
-        if (!d.active[c]) continue;
-        const int njs = d.item_njs[it];
+    {
         const int off = d.off[c], n = d.cnt[c];
         const int js = t % njs;
         const int il = t / njs;
-        const int i = d.item_i0[it] + il;  // local index in the cluster
+        const int i = i0 + il;  // local index in the cluster
         const bool ok = i < n;
         const int ip = off + (ok ? i : 0);
         // accumulators: hi/lo pairs of acc3 jerk3 pot
@@ -169,8 +187,9 @@ __global__ void __launch_bounds__(kWG) kc_force(Dev d)
         // kept as max(v^2/s) (squared at the end: squaring is monotonic,
         // rounding included) and max(m_ij^2 r^2/s^4)
         float cq1 = 0.0f, cq2 = 0.0f;
-        for (int tb = 0; tb < n; tb += kTJ) {
-            const int jn = min(kTJ, n - tb);
+        const int tj = d.tj_class[size_class_d(n)];
+        for (int tb = 0; tb < n; tb += tj) {
+            const int jn = min(tj, n - tb);
             __syncthreads();
             if (t < jn) {
                 const int q = off + tb + t;
@@ -292,17 +311,31 @@ __global__ void __launch_bounds__(kWG) kc_force(Dev d)
     }
 }
 
+// One force pass over the host-built item list: workgroups pull items off
+// the queue (reset by kc_predict) until it is empty.
+template <bool MIXED>
+__global__ void __launch_bounds__(kWG) kc_force(Dev d)
+{
+    __shared__ TileU tile;
+    __shared__ int s_item;
+    const int t = threadIdx.x;
+    for (;;) {
+        if (t == 0) s_item = atomicAdd(d.queue, 1);
+        __syncthreads();
+        const int it = s_item;
+        __syncthreads();
+        if (it >= d.nitems) break;
+        const int c = d.item_c[it];
+        if (!d.active[c]) continue;
+        force_item<MIXED>(d, tile, c, d.item_i0[it], d.item_njs[it]);
+    }
+}
+
 // ---------------------------------------------------------------------------
 // time-symmetric corrector (FP64), src/amuse_hermite/interface.cc:536-556
 // ---------------------------------------------------------------------------
-__global__ void __launch_bounds__(256) kc_correct(Dev d)
+__device__ __forceinline__ void correct_particle(const Dev &d, int p, double dt)
 {
-    const int q = blockIdx.x * blockDim.x + threadIdx.x;
-    if (q >= d.nlist) return;
-    const int p = d.plist[q];
-    const int c = d.pc[p];
-    if (!d.active[c]) return;
-    const double dt = d.dt[c];
 #pragma unroll
     for (int k = 0; k < 3; k++) {
         const double oa = d.a[k][p], oj = d.j[k][p], ov = d.v[k][p], ox = d.x[k][p];
@@ -313,6 +346,16 @@ __global__ void __launch_bounds__(256) kc_correct(Dev d)
         d.a[k][p] = na;
         d.j[k][p] = nj;
     }
+}
+
+__global__ void __launch_bounds__(256) kc_correct(Dev d)
+{
+    const int q = blockIdx.x * blockDim.x + threadIdx.x;
+    if (q >= d.nlist) return;
+    const int p = d.plist[q];
+    const int c = d.pc[p];
+    if (!d.active[c]) return;
+    correct_particle(d, p, d.dt[c]);
 }
 
 // first forces of each cluster become its current acc and jerk
@@ -341,10 +384,17 @@ __device__ __forceinline__ double next_step(const Dev &d, double t, unsigned int
     return s < d.dt_floor ? d.dt_floor : s;
 }
 
+__device__ __forceinline__ void update_cluster(const Dev &d, int c, int initial);
+
 __global__ void __launch_bounds__(256) kc_update(Dev d, int initial)
 {
     const int c = blockIdx.x * blockDim.x + threadIdx.x;
     if (c >= d.nc) return;
+    update_cluster(d, c, initial);
+}
+
+__device__ __forceinline__ void update_cluster(const Dev &d, int c, int initial)
+{
     const unsigned int cb = d.cmax[c];
     d.cmax[c] = 0u;
     if (initial) {
@@ -398,6 +448,185 @@ __global__ void kc_set_active_count(Dev d)
 }
 
 // ---------------------------------------------------------------------------
+// The whole run in one persistent launch (kc_run)
+// ---------------------------------------------------------------------------
+// A cooperative launch (every workgroup resident) repeats the four phases of
+// a step round -- predict, force, correct, update -- with a grid barrier
+// after each, until no cluster is active or the round budget is spent; both
+// tests read values that no workgroup writes until after the next barrier,
+// so every workgroup leaves the loop at the same round.  The work list is
+// built on the device by workgroup 0, as build_items does on the host:
+// the items of the active clusters (clusters largest first), rebuilt when
+// half the covered clusters have retired, with the lanes per i raised
+// ("boost") until the list holds fill_target items.
+
+// Loads of values other workgroups wrote before the last grid barrier.
+__device__ __forceinline__ int load_shared_int(const int *p)
+{
+    return __hip_atomic_load(const_cast<int *>(p), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+__device__ __forceinline__ int lanes_of(const Dev &d, int n, int boost)
+{
+    return min(64, d.njs_class[size_class_d(n)] * boost);
+}
+
+// Sum over the workgroup (kWG threads) of v, returned to every thread.
+__device__ __forceinline__ int wg_sum(int v, int *red)
+{
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o);
+    __syncthreads();
+    if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = v;
+    __syncthreads();
+    int s = 0;
+    for (int w = 0; w < kWG / 64; w++) s += red[w];
+    __syncthreads();
+    return s;
+}
+
+// Workgroup 0: the work list of the clusters active now.  fill = 0: every
+// cluster at its tuned lanes per i (boost 1), as at the start of a run.
+__device__ void build_work_list(const Dev &d, int fill, int *red)
+{
+    const int t = threadIdx.x;
+    const int per = (d.nc + kWG - 1) / kWG;  // clusters per thread, in corder
+    const int c0 = min(d.nc, t * per), c1 = min(d.nc, c0 + per);
+    int act = 0;
+    for (int k = c0; k < c1; k++) act += d.active[d.corder[k]] ? 1 : 0;
+    const int covered = wg_sum(act, red);
+    int boost = 1;
+    if (fill) {
+        const int target = max(1, d.fill_target);
+        for (; boost < 64; boost *= 2) {
+            int count = 0, room = 0;
+            for (int k = c0; k < c1; k++) {
+                const int c = d.corder[k];
+                if (!d.active[c]) continue;
+                const int n = d.cnt[c], njs = lanes_of(d, n, boost);
+                const int rows = kWG / njs;
+                count += (n + rows - 1) / rows;
+                room |= njs < 64 ? 1 : 0;
+            }
+            count = wg_sum(count, red);
+            room = wg_sum(room, red);
+            if (count >= target || room == 0) break;
+        }
+    }
+    // items of this thread's clusters, then their offsets (exclusive scan)
+    int mine = 0;
+    for (int k = c0; k < c1; k++) {
+        const int c = d.corder[k];
+        if (!d.active[c]) continue;
+        const int n = d.cnt[c], rows = kWG / lanes_of(d, n, boost);
+        mine += (n + rows - 1) / rows;
+    }
+    int incl = mine;
+    const int lane = t & 63;
+    for (int o = 1; o < 64; o <<= 1) {
+        const int u = __shfl_up(incl, o);
+        if (lane >= o) incl += u;
+    }
+    __syncthreads();
+    if (lane == 63) red[t >> 6] = incl;
+    __syncthreads();
+    int base = 0;
+    for (int w = 0; w < (t >> 6); w++) base += red[w];
+    int at = base + incl - mine;
+    int total = 0;
+    for (int w = 0; w < kWG / 64; w++) total += red[w];
+    for (int k = c0; k < c1; k++) {
+        const int c = d.corder[k];
+        if (!d.active[c]) continue;
+        const int n = d.cnt[c], njs = lanes_of(d, n, boost), rows = kWG / njs;
+        for (int i0 = 0; i0 < n && at < d.wl_cap; i0 += rows, at++) {
+            d.wl_c[at] = c;
+            d.wl_i0[at] = i0;
+            d.wl_njs[at] = njs;
+        }
+    }
+    if (t == 0) {
+        d.ctl[0] = min(total, d.wl_cap);
+        d.ctl[1] = covered;
+        d.ctl[2] = boost;
+        d.ctl[3] = 1;
+    }
+    __syncthreads();
+}
+
+// Rows of work item `it`: the particles of (cluster, i0 .. i0 + kWG/njs - 1).
+__device__ __forceinline__ int item_particle(const Dev &d, int it, int t, int &c)
+{
+    c = d.wl_c[it];
+    const int i = d.wl_i0[it] + t;
+    if (t >= kWG / d.wl_njs[it] || i >= d.cnt[c]) return -1;
+    return d.off[c] + i;
+}
+
+template <bool MIXED>
+__global__ void __launch_bounds__(kWG) kc_run(Dev d, long long max_rounds, long long *rounds_out)
+{
+    __shared__ TileU tile;
+    __shared__ int s_item;
+    __shared__ int red[kWG / 64];
+    namespace cg = cooperative_groups;
+    cg::grid_group grid = cg::this_grid();
+    const int t = threadIdx.x;
+    long long r = 0;
+    // The list state as every workgroup keeps it in registers: read by all
+    // before the barrier, written (by workgroup 0, in a rebuild) only after
+    // it, and read again after each rebuild's own barrier.
+    int built = load_shared_int(&d.ctl[3]);
+    int covered = load_shared_int(&d.ctl[1]);
+    grid.sync();
+    for (;;) {
+        // work list: none yet, or half the covered clusters retired (nactive
+        // changes only in the update phase, three barriers below)
+        const int nact = load_shared_int(d.nactive);
+        if (!built || nact * 2 <= covered) {
+            if (blockIdx.x == 0) build_work_list(d, built, red);
+            grid.sync();
+            built = 1;
+            covered = load_shared_int(&d.ctl[1]);
+        }
+        const int nwork = load_shared_int(&d.ctl[0]);
+        // predict (and reset the force queue)
+        if (blockIdx.x == 0 && t == 0) *d.queue = 0;
+        for (int it = blockIdx.x; it < nwork; it += gridDim.x) {
+            int c;
+            const int p = item_particle(d, it, t, c);
+            if (p >= 0 && d.active[c]) predict_particle(d, p, d.dt[c]);
+        }
+        grid.sync();
+        // force: items off the queue
+        for (;;) {
+            if (t == 0) s_item = atomicAdd(d.queue, 1);
+            __syncthreads();
+            const int it = s_item;
+            __syncthreads();
+            if (it >= nwork) break;
+            const int c = d.wl_c[it];
+            if (!d.active[c]) continue;
+            force_item<MIXED>(d, tile, c, d.wl_i0[it], d.wl_njs[it]);
+        }
+        grid.sync();
+        // correct
+        for (int it = blockIdx.x; it < nwork; it += gridDim.x) {
+            int c;
+            const int p = item_particle(d, it, t, c);
+            if (p >= 0 && d.active[c]) correct_particle(d, p, d.dt[c]);
+        }
+        grid.sync();
+        // update: time, next step, retire
+        for (int c = blockIdx.x * kWG + t; c < d.nc; c += gridDim.x * kWG)
+            update_cluster(d, c, 0);
+        grid.sync();
+        r++;
+        if (load_shared_int(d.nactive) <= 0 || r >= max_rounds) break;
+    }
+    if (blockIdx.x == 0 && t == 0) *rounds_out = r;
+}
+
+// ---------------------------------------------------------------------------
 // host side
 // ---------------------------------------------------------------------------
 struct Ensemble {
@@ -406,24 +635,19 @@ struct Ensemble {
     void *base = nullptr;
     int precision = 1;
     std::vector<int> cnt, off;
-    int njs_class[kClasses];   // tuned lanes-per-i for each size class
     int ib_rows = 0;
     int persistent_wgs = 0;
     hipGraphExec_t graph = nullptr;
     int graph_steps = 0;
     int graph_prec = -1;
     int items_clusters = 0;  // clusters covered by the current work items
+    int *wl_base = nullptr;  // the persistent run's work list and cluster order
     double force_ms = 0.0;
     long long launches = 0;
     std::string err;
 };
 
-int size_class(int n)
-{
-    int k = 0, lim = 16;
-    while (n > lim && k < kClasses - 1) { lim *= 2; k++; }
-    return k;
-}
+int size_class(int n) { return size_class_d(n); }
 
 #define CK(expr)                                                       \
     do {                                                               \
@@ -448,7 +672,7 @@ int build_items(Ensemble *E, const int *only = nullptr, bool fill = false)
     // the previous item arrays may still be read by queued work
     CK(hipStreamSynchronize(E->stream));
     auto njs_of = [&](int n, int boost) {
-        return std::min(64, E->njs_class[size_class(n)] * boost);
+        return std::min(64, E->d.njs_class[size_class(n)] * boost);
     };
     int boost = 1;
     if (fill) {
@@ -570,7 +794,7 @@ G6C_API void *g6c_create(int nc, const int counts[], const double m[], const dou
     d.np = int(np);
     d.eps2 = eps2;
     d.dt_param = dt_param > 0.0 ? dt_param : 0.03;
-    for (int k = 0; k < kClasses; k++) E->njs_class[k] = k <= 1 ? 16 : (k <= 3 ? 4 : 1);
+    for (int k = 0; k < kClasses; k++) E->d.njs_class[k] = k <= 1 ? 16 : (k <= 3 ? 4 : 1);
     if (hipStreamCreateWithFlags(&E->stream, hipStreamNonBlocking) != hipSuccess) { delete E; return nullptr; }
     // one allocation: per-particle doubles (x v a j px pv na nj = 24, m, npot), floats (10),
     // ints (pc plist), per-cluster doubles (t dt energy) + uint (cmax) + ints (off cnt active nsteps)
@@ -630,12 +854,41 @@ G6C_API void *g6c_create(int nc, const int counts[], const double m[], const dou
     (void)hipGetDevice(&dev);
     (void)hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
     E->persistent_wgs = cus * 4;
+    d.fill_target = std::max(1, E->persistent_wgs / 2);
+    // work list of the persistent run: at least 4 rows per item, so at most
+    // n/4 + 1 items per cluster
+    d.wl_cap = int(std::min<long long>(P / 4 + Cn + 8, 1ll << 30));
+    d.ctl = d.queue + 2;
+    std::vector<int> order(nc);
+    std::iota(order.begin(), order.end(), 0);
+    std::stable_sort(order.begin(), order.end(), [&](int a, int b) { return counts[a] > counts[b]; });
+    if (ok) ok = hipMalloc(reinterpret_cast<void **>(&E->wl_base),
+                           sizeof(int) * (3 * size_t(d.wl_cap) + Cn)) == hipSuccess;
+    if (ok) {
+        d.wl_c = E->wl_base;
+        d.wl_i0 = d.wl_c + d.wl_cap;
+        d.wl_njs = d.wl_i0 + d.wl_cap;
+        d.corder = d.wl_njs + d.wl_cap;
+        ok &= hipMemcpy(d.corder, order.data(), Cn * 4, hipMemcpyHostToDevice) == hipSuccess;
+        const int zero8[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+        ok &= hipMemcpy(d.ctl, zero8, sizeof(zero8), hipMemcpyHostToDevice) == hipSuccess;
+    }
     if (!ok || build_items(E)) {
+        if (E->wl_base) (void)hipFree(E->wl_base);
         (void)hipFree(E->base);
         delete E;
         return nullptr;
     }
     return E;
+}
+
+// The device-built work list no longer matches the tiles or the active set.
+static int invalidate_work_list(Ensemble *E)
+{
+    const int zero = 0;
+    CK(hipMemcpyAsync(E->d.ctl + 3, &zero, sizeof(int), hipMemcpyHostToDevice, E->stream));
+    CK(hipStreamSynchronize(E->stream));
+    return 0;
 }
 
 G6C_API const char *g6c_last_error(void *h) { return static_cast<Ensemble *>(h)->err.c_str(); }
@@ -666,7 +919,43 @@ G6C_API int g6c_start(void *h, double t_end)
     hipLaunchKernelGGL(kc_update, dim3(cb), dim3(256), 0, E->stream, E->d, 1);
     hipLaunchKernelGGL(kc_set_active_count, dim3(1), dim3(1), 0, E->stream, E->d);
     CK(hipStreamSynchronize(E->stream));
-    return 0;
+    return invalidate_work_list(E);
+}
+
+// use_graph == 2: up to max_rounds step rounds (0: until every cluster has
+// reached t_end) in persistent cooperative launches of kc_run, one unless a
+// launch's round budget (2^20 rounds) runs out first.
+static long long run_persistent(Ensemble *E, long long max_rounds)
+{
+    int dev = 0, cus = 0, per_cu = 0;
+    CK(hipGetDevice(&dev));
+    CK(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
+    const void *kern = E->precision == G6X_MIXED ? reinterpret_cast<const void *>(&kc_run<true>)
+                                                 : reinterpret_cast<const void *>(&kc_run<false>);
+    CK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kern, kWG, 0));
+    if (per_cu <= 0) {
+        E->err = "kc_run: no workgroup fits a compute unit";
+        return G6X_ERR_HIP;
+    }
+    const int grid = per_cu * cus;
+    long long *d_rounds = reinterpret_cast<long long *>(E->d.ctl + 4);
+    long long total = 0;
+    for (;;) {
+        long long budget = 1ll << 20;
+        if (max_rounds > 0) budget = std::min(budget, max_rounds - total);
+        if (budget <= 0) break;
+        void *args[] = {&E->d, &budget, &d_rounds};
+        CK(hipLaunchCooperativeKernel(kern, dim3(grid), dim3(kWG), args, 0, E->stream));
+        long long done = 0;
+        int nact = 0;
+        CK(hipMemcpyAsync(&done, d_rounds, sizeof(done), hipMemcpyDeviceToHost, E->stream));
+        CK(hipMemcpyAsync(&nact, E->d.nactive, sizeof(nact), hipMemcpyDeviceToHost, E->stream));
+        CK(hipStreamSynchronize(E->stream));
+        total += done;
+        E->launches++;
+        if (nact <= 0) break;
+    }
+    return total;
 }
 
 static int capture_graph(Ensemble *E, int steps_per_graph)
@@ -695,6 +984,21 @@ G6C_API long long g6c_run(void *h, int steps_per_graph, int use_graph, long long
 {
     Ensemble *E = static_cast<Ensemble *>(h);
     if (steps_per_graph < 1) steps_per_graph = 1;
+    if (use_graph == 2) {
+        hipEvent_t a, b;
+        CK(hipEventCreate(&a));
+        CK(hipEventCreate(&b));
+        (void)hipEventRecord(a, E->stream);
+        const long long r = run_persistent(E, max_rounds);
+        (void)hipEventRecord(b, E->stream);
+        (void)hipEventSynchronize(b);
+        float ms = 0;
+        (void)hipEventElapsedTime(&ms, a, b);
+        E->force_ms += ms;
+        (void)hipEventDestroy(a);
+        (void)hipEventDestroy(b);
+        return r;
+    }
     int *h_active = nullptr;
     CK(hipHostMalloc(reinterpret_cast<void **>(&h_active), sizeof(int)));
     std::vector<int> flags(E->d.nc);
@@ -815,19 +1119,41 @@ G6C_API int g6c_set_tiles(void *h, const int *njs_per_class)
         const int v = njs_per_class[k];
         if (v < 1 || v > 64 || (v & (v - 1))) return G6X_ERR_ARG;
     }
-    std::copy(njs_per_class, njs_per_class + kClasses, E->njs_class);
+    std::copy(njs_per_class, njs_per_class + kClasses, E->d.njs_class);
+    if (invalidate_work_list(E)) return G6X_ERR_HIP;
     return build_items(E);
+}
+
+// j per LDS tile of each size class (16 .. 256, a power of two).
+G6C_API int g6c_set_jtiles(void *h, const int *tj_per_class)
+{
+    Ensemble *E = static_cast<Ensemble *>(h);
+    for (int k = 0; k < kClasses; k++) {
+        const int v = tj_per_class[k];
+        if (v < 16 || v > kTJ || (v & (v - 1))) return G6X_ERR_ARG;
+    }
+    std::copy(tj_per_class, tj_per_class + kClasses, E->d.tj_class);
+    return 0;
+}
+
+G6C_API int g6c_get_jtiles(void *h, int *tj_per_class)
+{
+    Ensemble *E = static_cast<Ensemble *>(h);
+    std::copy(E->d.tj_class, E->d.tj_class + kClasses, tj_per_class);
+    return kClasses;
 }
 
 G6C_API int g6c_get_tiles(void *h, int *njs_per_class)
 {
     Ensemble *E = static_cast<Ensemble *>(h);
-    std::copy(E->njs_class, E->njs_class + kClasses, njs_per_class);
+    std::copy(E->d.njs_class, E->d.njs_class + kClasses, njs_per_class);
     return kClasses;
 }
 
 // Autotuner: for each size class present, times the force pass restricted
-// to that class's clusters for njs in {1, 2, ..., 64} and keeps the fastest.
+// to that class's clusters for every (lanes per i, j per tile) pair -- njs
+// in {1, 2, ..., 64} (i-block rows 256/njs), tile 64, 128 or 256 j where
+// shorter than the class's largest cluster -- and keeps the fastest.
 G6C_API int g6c_autotune(void *h)
 {
     Ensemble *E = static_cast<Ensemble *>(h);
@@ -844,31 +1170,42 @@ G6C_API int g6c_autotune(void *h)
             if (size_class(E->cnt[c]) == k) { act[c] = 1; any = true; }
         if (!any) continue;
         CK(hipMemcpy(E->d.active, act.data(), sizeof(int) * E->d.nc, hipMemcpyHostToDevice));
+        int nmax = 0;
+        for (int c = 0; c < E->d.nc; c++)
+            if (act[c]) nmax = std::max(nmax, E->cnt[c]);
         float best = 1e30f;
-        int best_njs = E->njs_class[k];
-        for (int njs = 1; njs <= 64; njs *= 2) {
-            E->njs_class[k] = njs;
-            if (build_items(E, act.data())) return G6X_ERR_HIP;
-            hipLaunchKernelGGL(kc_predict, dim3(list_blocks(E)), dim3(256), 0, E->stream, E->d, 1);
-            launch_force(E, E->precision);
-            (void)hipEventRecord(a, E->stream);
-            for (int r = 0; r < 3; r++) {
+        int best_njs = E->d.njs_class[k], best_tj = E->d.tj_class[k];
+        // tiles shorter than the largest cluster of the class (a longer one
+        // holds the whole cluster, like kTJ)
+        for (int tj = kTJ; tj >= 64; tj /= 2) {
+            if (tj < kTJ && tj >= nmax) continue;
+            E->d.tj_class[k] = tj;
+            for (int njs = 1; njs <= 64; njs *= 2) {
+                E->d.njs_class[k] = njs;
+                if (build_items(E, act.data())) return G6X_ERR_HIP;
                 hipLaunchKernelGGL(kc_predict, dim3(list_blocks(E)), dim3(256), 0, E->stream, E->d, 1);
                 launch_force(E, E->precision);
+                (void)hipEventRecord(a, E->stream);
+                for (int r = 0; r < 3; r++) {
+                    hipLaunchKernelGGL(kc_predict, dim3(list_blocks(E)), dim3(256), 0, E->stream, E->d, 1);
+                    launch_force(E, E->precision);
+                }
+                (void)hipEventRecord(b, E->stream);
+                CK(hipEventSynchronize(b));
+                float ms = 0;
+                (void)hipEventElapsedTime(&ms, a, b);
+                timed++;
+                if (ms < best) { best = ms; best_njs = njs; best_tj = tj; }
             }
-            (void)hipEventRecord(b, E->stream);
-            CK(hipEventSynchronize(b));
-            float ms = 0;
-            (void)hipEventElapsedTime(&ms, a, b);
-            timed++;
-            if (ms < best) { best = ms; best_njs = njs; }
         }
-        E->njs_class[k] = best_njs;
+        E->d.njs_class[k] = best_njs;
+        E->d.tj_class[k] = best_tj;
     }
     (void)hipEventDestroy(a);
     (void)hipEventDestroy(b);
     if (build_items(E)) return G6X_ERR_HIP;
     CK(hipMemcpy(E->d.active, saved.data(), sizeof(int) * E->d.nc, hipMemcpyHostToDevice));
+    if (invalidate_work_list(E)) return G6X_ERR_HIP;
     return timed;
 }
 
@@ -889,6 +1226,7 @@ G6C_API void g6c_free(void *h)
     (void)hipStreamSynchronize(E->stream);
     if (E->graph) (void)hipGraphExecDestroy(E->graph);
     if (E->d.item_c) (void)hipFree(E->d.item_c);
+    if (E->wl_base) (void)hipFree(E->wl_base);
     if (E->base) (void)hipFree(E->base);
     (void)hipStreamDestroy(E->stream);
     delete E;

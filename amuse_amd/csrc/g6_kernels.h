@@ -224,10 +224,38 @@ bool force_cfg_supported(int njs, int ipl);
 // pred_j: the j-slots are predicted to ti (compat mode as given, k18 = 0)
 // from their base records as the kernel loads them, and the predicted store
 // is neither read nor written; only for pred_force_cfg launches.
+// self_i0 >= 0: i row r is j-slot self_i0 + r and the nj slots hold
+// distinct ids, none empty, so work items whose j-range holds none of their
+// i-slots skip the per-pair id compare; -1: every pair compares ids.
+// tbox != nullptr: the slots are in spatial order and tbox holds each
+// kTJ-slot tile's position box (spatial_order); mixed path, shapes of
+// spatial_cfg_supported only.
 hipError_t launch_force(const JStore &s, const IBlock &b, int ni, int nj,
                         double eps2, int prec, bool want_nn,
                         const ForceCfg &cfg, hipStream_t st, bool pred_j = false,
-                        double ti = 0.0, int compat = 0);
+                        double ti = 0.0, int compat = 0, int self_i0 = -1,
+                        const float *tbox = nullptr);
+bool spatial_cfg_supported(const ForceCfg &c);
+
+// Spatial order of a full pass (g6_spatial.hip): the mixed predicted slots
+// [0, nj) of `s` (all holding distinct ids) sorted along a Morton curve of
+// their positions into `o` (mixed predicted arrays, id and pid), perm[k] =
+// the slot at sorted position k, tbox[6 * t ...] = min xyz, max xyz of sorted
+// tile t (kTJ slots).  SpatialBuf holds the keys and the sort's temporary
+// storage, grown on demand (the only allocations).
+struct SpatialBuf {
+    void *base = nullptr;
+    size_t bytes = 0;
+    void *tmp = nullptr;
+    size_t tmp_bytes = 0;
+};
+hipError_t spatial_order(const JStore &s, int nj, const JStore &o, int *perm, float *tbox,
+                         SpatialBuf &buf, hipStream_t st);
+// Result rows of a spatially ordered pass (pitch S_in, sorted order) to slot
+// order (pitch S_out): out[k][perm[i]] = in[k][i], nn likewise.
+hipError_t launch_unsort_rows(const double *in, const int *in_nn, int S_in, const int *perm,
+                              int ni, double *out, int *out_nn, int S_out, bool nn,
+                              hipStream_t st);
 // Launches that can predict their j-slots as they load them: one i per
 // wave (64 lanes), one j-chunk, at most 8 tiles.
 inline bool pred_force_cfg(const ForceCfg &c, int nj)

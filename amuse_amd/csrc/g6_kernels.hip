@@ -215,8 +215,11 @@ __device__ __forceinline__ double exact_r2(double dx, double dy, double dz)
 }
 
 struct AccD {
-    double ax, ay, az, jx, jy, jz, pot, best;
-    int addr;
+    double ax, ay, az, jx, jy, jz, pot;
+    double best;   // nearest-neighbour r^2 so far (blocks before the current one)
+    int addr;      // its slot (NNM 2: found at the end, from blo/bhi)
+    double tbest;  // NNM 2: running minimum of r^2 over the current block of j
+    int blo, bhi;  // NNM 2: slot range of the lane's block that holds best
 };
 
 struct ID {
@@ -244,24 +247,30 @@ __device__ __forceinline__ void flush_f64(AccD &a, DDAcc &d)
     a.ax = a.ay = a.az = a.jx = a.jy = a.jz = a.pot = 0.0;
 }
 
-template <bool NN>
+// NNM: 0 no nearest neighbour; 1 per-pair compare, first slot wins; 2 a
+// running minimum of r^2 over a block of j, the slot found at the end (as
+// pair_mx).  CHK: pairs with equal ids are skipped (y = 0, never the nearest
+// neighbour); without it the caller knows no j of the item shares an id
+// with an i of it.
+template <int NNM, bool CHK = true>
 __device__ __forceinline__ void pair_f64(const ID &I, AccD &a, double xj, double yj, double zj,
                                          double mj, double vxj, double vyj, double vzj, int idj,
                                          int addr, double eps2)
 {
     const double dx = xj - I.x, dy = yj - I.y, dz = zj - I.z;
     // r^2 and s = r^2 + eps2 rounded as lib/g6lib/g6lib.c:149,156 does, in
-    // both variants, so forces do not depend on whether nn was asked for
+    // every variant, so forces do not depend on whether nn was asked for
     const double r2 = exact_r2(dx, dy, dz);
     const double s = r2 + eps2;
-    const bool valid = idj != I.id;
+    const bool valid = !CHK || idj != I.id;
     double y = rsqrt_f64(s);
-    y = valid ? y : 0.0;
-    if (NN) {
+    if (CHK) y = valid ? y : 0.0;
+    if (NNM == 1) {
         const bool better = valid && (r2 < a.best);
         a.best = better ? r2 : a.best;
         a.addr = better ? addr : a.addr;
     }
+    if (NNM == 2) a.tbest = __builtin_fmin(a.tbest, valid ? r2 : __builtin_inf());
     const double y2 = y * y;
     const double mi = mj * y;
     const double mi3 = mi * y2;
@@ -299,7 +308,33 @@ struct FParams {
     const double *bx[3], *bv[3], *ba2[3], *bj6[3], *btj;
     double tpred;
     int compat;
+    // >= 0: i row r is j-slot self_i0 + r, and no two of the nj slots share
+    // an id (none is empty), so only an item whose j-range holds one of its
+    // i-slots can meet a pair with equal ids; -1: ids compared in every pair.
+    int self_i0 = -1;
+    // spatially ordered full pass: per j-tile (kTJ slots) the box of the
+    // positions, min xyz then max xyz (k_force_mx_sp)
+    const float *tbox = nullptr;
 };
+
+// Lanes' j per item from which the nearest neighbour is kept as a running
+// minimum per block of a lane's j with one slot search at the end (below: a
+// compare per pair).
+constexpr int kLongItem = 16384;
+// a lane's j per such block in the FP64 kernels (the final search gives one
+// lane of a wave to each j of a block)
+constexpr int kBlockJ = 64;
+
+// Whether an item (i rows [r0, r0 + rows), j-chunk by) must compare ids.
+__device__ __forceinline__ bool needs_id_check(const FParams &p, int r0, int rows, int by)
+{
+    if (p.self_i0 < 0) return true;
+    const int j0 = by * p.jchunk;
+    const int j1 = min(p.nj, j0 + p.jchunk);
+    const int s0 = p.self_i0 + r0;
+    const int s1 = p.self_i0 + min(p.ni, r0 + rows);
+    return s0 < j1 && j0 < s1;
+}
 
 // j-slot j predicted to p.tpred from its base record, as the predictor
 // kernels do (k18 = 0); the caller converts it as write_pred does.
@@ -328,10 +363,14 @@ struct TileF {
 // per workgroup, kd_* below loops over items whose count lives on the device.
 // PRED: the j-slots are predicted to p.tpred as the tiles are loaded (the
 // values write_pred would store), not read from the predicted store.
-template <int NJS, int IPL, bool NN, bool PRED = false>
+// LONG (with NN): the nearest neighbour is a running minimum per block of
+// kBlockJ of a lane's j, its slot found once at the end (as force_mx_item);
+// otherwise a compare and selects per pair.  CHK: as in pair_f64.
+template <int NJS, int IPL, bool NN, bool LONG = false, bool PRED = false, bool CHK = true>
 __device__ __forceinline__ void force_f64_item(const FParams &p, TileD &tile, int bx, int by,
                                        int nchunk)
 {
+    constexpr int NNM = NN ? (LONG ? 2 : 1) : 0;
     constexpr int IBL = kWG / NJS;
     constexpr int IB = IBL * IPL;
     // tiles between two folds of the plain sums: >= 128 pairs per lane
@@ -351,7 +390,7 @@ __device__ __forceinline__ void force_f64_item(const FParams &p, TileD &tile, in
         I[q].x = p.ix[i]; I[q].y = p.iy[i]; I[q].z = p.iz[i];
         I[q].vx = p.ivx[i]; I[q].vy = p.ivy[i]; I[q].vz = p.ivz[i];
         I[q].id = p.iid[i];
-        A[q] = AccD{0, 0, 0, 0, 0, 0, 0, __builtin_inf(), -1};
+        A[q] = AccD{0, 0, 0, 0, 0, 0, 0, __builtin_inf(), -1, __builtin_inf(), 0, 0};
 #pragma unroll
         for (int k = 0; k < 7; k++) { D[q].h[k] = 0.0; D[q].l[k] = 0.0; }
     }
@@ -381,13 +420,30 @@ __device__ __forceinline__ void force_f64_item(const FParams &p, TileD &tile, in
             }
         }
         __syncthreads();
+        for (int jb = js; jb < jn; jb += NJS * kBlockJ) {
+            const int je = min(jn, jb + NJS * kBlockJ);
 #pragma unroll G6_F64_UNROLL
-        for (int jj = js; jj < jn; jj += NJS) {
-            const double2 a = tile.xy[jj], b = tile.zm[jj], c = tile.vxy[jj], d = tile.vzid[jj];
-            const int idj = int(__double_as_longlong(d.y));
+            for (int jj = jb; jj < je; jj += NJS) {
+                const double2 a = tile.xy[jj], b = tile.zm[jj], c = tile.vxy[jj],
+                              d = tile.vzid[jj];
+                const int idj = int(__double_as_longlong(d.y));
 #pragma unroll
-            for (int q = 0; q < IPL; q++)
-                pair_f64<NN>(I[q], A[q], a.x, a.y, b.x, b.y, c.x, c.y, d.x, idj, tb + jj, eps2);
+                for (int q = 0; q < IPL; q++)
+                    pair_f64<NNM, CHK>(I[q], A[q], a.x, a.y, b.x, b.y, c.x, c.y, d.x, idj,
+                                       tb + jj, eps2);
+            }
+            if (NNM == 2) {
+                // a block minimum below the best so far (strictly: an equal
+                // r^2 keeps the earlier block) moves the best to this block
+#pragma unroll
+                for (int q = 0; q < IPL; q++) {
+                    const bool imp = A[q].tbest < A[q].best;
+                    A[q].best = imp ? A[q].tbest : A[q].best;
+                    A[q].blo = imp ? tb + jb : A[q].blo;
+                    A[q].bhi = imp ? tb + je : A[q].bhi;
+                    A[q].tbest = __builtin_inf();
+                }
+            }
         }
         if (++tcount == FT) {
             tcount = 0;
@@ -397,6 +453,33 @@ __device__ __forceinline__ void force_f64_item(const FParams &p, TileD &tile, in
     }
 #pragma unroll
     for (int q = 0; q < IPL; q++) flush_f64(A[q], D[q]);
+    if (NNM == 2) {
+        // The slot: the first j of the best block, in the lane's stride, with
+        // exactly that r^2 (recomputed from the arrays the tiles were loaded
+        // from, so bit-identical); the wave searches one lane's block at a
+        // time, lane k taking the block's k-th j, and the first match is the
+        // lowest set bit of the ballot.
+        const int lane = threadIdx.x & 63;
+#pragma unroll
+        for (int q = 0; q < IPL; q++) {
+            for (int L = 0; L < 64; L++) {
+                const double best = __shfl(A[q].best, L);
+                if (!(best < __builtin_inf())) continue;  // wave-uniform
+                const int blo = __shfl(A[q].blo, L), bhi = __shfl(A[q].bhi, L);
+                const double xi = __shfl(I[q].x, L), yi = __shfl(I[q].y, L),
+                             zi = __shfl(I[q].z, L);
+                const int idi = __shfl(I[q].id, L);
+                const int j = blo + lane * NJS;
+                bool hit = false;
+                if (j < bhi) {
+                    const double r2 = exact_r2(p.jx[j] - xi, p.jy[j] - yi, p.jz[j] - zi);
+                    hit = r2 == best && (!CHK || p.jid[j] != idi);
+                }
+                const unsigned long long mask = __ballot(hit);
+                if (lane == L && mask) A[q].addr = blo + (__ffsll((long long)mask) - 1) * NJS;
+            }
+        }
+    }
 
     // Combine the NJS lanes of each i (double-double adds).
 #pragma unroll
@@ -474,11 +557,17 @@ __device__ __forceinline__ void force_f64_item(const FParams &p, TileD &tile, in
     }
 }
 
-template <int NJS, int IPL, bool NN>
+// LONG (with NN) is chosen by the host (long_items): one kernel holding all
+// four (LONG, id compare) forms needs ~220 VGPRs against ~166 for two.
+template <int NJS, int IPL, bool NN, bool LONG>
 __global__ void __launch_bounds__(kWG) k_force_f64(FParams p)
 {
     __shared__ TileD tile;
-    force_f64_item<NJS, IPL, NN>(p, tile, blockIdx.x, blockIdx.y, gridDim.y);
+    if (needs_id_check(p, blockIdx.x * (kWG / NJS) * IPL, (kWG / NJS) * IPL, blockIdx.y))
+        force_f64_item<NJS, IPL, NN, LONG>(p, tile, blockIdx.x, blockIdx.y, gridDim.y);
+    else
+        force_f64_item<NJS, IPL, NN, LONG, false, false>(p, tile, blockIdx.x, blockIdx.y,
+                                                         gridDim.y);
 }
 
 // Short j-ranges with the j-slots predicted as they are loaded (one i per
@@ -487,7 +576,7 @@ template <bool NN>
 __global__ void __launch_bounds__(kWG) k_force_f64_p(FParams p)
 {
     __shared__ TileD tile;
-    force_f64_item<64, 1, NN, true>(p, tile, blockIdx.x, 0, 1);
+    force_f64_item<64, 1, NN, false, true>(p, tile, blockIdx.x, 0, 1);
 }
 
 // ------------------------------------------------------------------------
@@ -512,9 +601,11 @@ struct DSAcc {
 
 
 // Difference vector and r^2 of one pair, mixed path: dx from double-single
-// positions, r^2 = (dx dx + dy dy) + dz dz with FMAs.  The self pair gets
-// r^2 = +inf, so s = +inf, r^-1/2 = 0 and every term is exactly zero, and
-// it can never be the nearest neighbour.
+// positions, r^2 = (dx dx + dy dy) + dz dz with FMAs.  CHK: the self pair
+// (same id) gets r^2 = +inf, so s = +inf, r^-1/2 = 0 and every term is
+// exactly zero, and it can never be the nearest neighbour; without CHK the
+// caller knows no j of the item shares an id with an i of it.
+template <bool CHK = true>
 __device__ __forceinline__ float pair_r2_mx(const IF &I, float4 ja, float4 jb, float4 jc,
                                             float &dx, float &dy, float &dz)
 {
@@ -522,6 +613,7 @@ __device__ __forceinline__ float pair_r2_mx(const IF &I, float4 ja, float4 jb, f
     dy = (ja.y - I.hy) + (jb.y - I.ly);
     dz = (ja.z - I.hz) + (jb.z - I.lz);
     const float r2 = __builtin_fmaf(dz, dz, __builtin_fmaf(dy, dy, dx * dx));
+    if (!CHK) return r2;
     return __float_as_int(jc.z) != I.id ? r2 : __builtin_inff();
 }
 
@@ -529,12 +621,12 @@ __device__ __forceinline__ float pair_r2_mx(const IF &I, float4 ja, float4 jb, f
 // running minimum of r^2 over a block of j: the caller keeps the block that
 // first reached the overall minimum and finds the slot in it once, at the
 // end, which takes the compare and two selects out of the pair.
-template <int NNM>
+template <int NNM, bool CHK = true>
 __device__ __forceinline__ void pair_mx(const IF &I, AccF &a, float4 ja, float4 jb, float4 jc,
                                         int addr, float eps2)
 {
     float dx, dy, dz;
-    const float r2 = pair_r2_mx(I, ja, jb, jc, dx, dy, dz);
+    const float r2 = pair_r2_mx<CHK>(I, ja, jb, jc, dx, dy, dz);
     // same s in every variant (forces independent of the nn request)
     const float s = r2 + eps2;
     const float y = __builtin_amdgcn_rsqf(s);
@@ -579,136 +671,17 @@ constexpr int kFlush = G6_MX_FLUSH;
 // the final nearest-neighbour search gives one lane to each j of a block
 static_assert(kFlush <= 64, "a block of a lane's j must fit one wave");
 
-// Lanes' j per item from which the nearest neighbour is kept as a running
-// minimum with one slot search at the end (below: a compare per pair).
-constexpr int kLongItem = 16384;
 
-// One (i-block bx, j-chunk by) work item; the kernel below runs one item
-// per workgroup, kd_* below loops over items whose count lives on the device.
-// LONG (with NN): a lane's j in the item number >= kLongItem, so the
-// nearest neighbour is a running minimum per block of kFlush j with the slot
-// found once at the end; otherwise a compare and two selects per pair, as a
-// final search (up to kFlush dependent loads) would dominate a short item.
-// Both give the first slot with the lane's minimal r^2.
-// PRED: as in force_f64_item.
-template <int NJS, int IPL, bool NN, bool LONG, bool PRED = false>
-__device__ __forceinline__ void force_mx_item(const FParams &p, TileF &tile, int bx, int by,
-                                       int nchunk)
+// End of a mixed work item: the NJS lanes of each i combined (double-single
+// adds, across waves through LDS when NJS > 64), then the sums and nearest
+// neighbour written as results (one chunk) or as the chunk's partial record.
+template <int NJS, int IPL, bool NN>
+__device__ __forceinline__ void mx_item_epilogue(const FParams &p, TileF &tile, IF *I, AccF *A,
+                                                 DSAcc *D, int ibase, int il, int js, int by,
+                                                 int nchunk)
 {
-    constexpr int NNM = NN ? (LONG ? 2 : 1) : 0;
     constexpr int IBL = kWG / NJS;
-    constexpr int IB = IBL * IPL;
-
     const int t = threadIdx.x;
-    const int js = t % NJS;
-    const int il = t / NJS;
-    const int ibase = bx * IB;
-
-    IF I[IPL];
-    AccF A[IPL];
-    DSAcc D[IPL];
-#pragma unroll
-    for (int q = 0; q < IPL; q++) {
-        const int i = min(ibase + il + q * IBL, p.ni - 1);
-        const double x = p.ix[i], y = p.iy[i], z = p.iz[i];
-        I[q].hx = float(x); I[q].lx = float(x - double(I[q].hx));
-        I[q].hy = float(y); I[q].ly = float(y - double(I[q].hy));
-        I[q].hz = float(z); I[q].lz = float(z - double(I[q].hz));
-        I[q].vx = float(p.ivx[i]); I[q].vy = float(p.ivy[i]); I[q].vz = float(p.ivz[i]);
-        I[q].id = p.iid[i];
-        A[q] = AccF{0, 0, 0, 0, 0, 0, 0, __builtin_inff(), -1, __builtin_inff(), 0, 0};
-#pragma unroll
-        for (int k = 0; k < 7; k++) { D[q].h[k] = 0.0f; D[q].l[k] = 0.0f; }
-    }
-    const float eps2 = float(p.eps2);
-    const int j0 = by * p.jchunk;
-    const int j1 = min(p.nj, j0 + p.jchunk);
-
-    for (int tb = j0; tb < j1; tb += kTJ) {
-        const int jn = min(kTJ, j1 - tb);
-        __syncthreads();
-        if (t < jn) {
-            const int j = tb + t;
-            if constexpr (PRED) {
-                const Pred pr = predict_base(p, j);
-                const bool empty = p.jid[j] == -1;
-                float hi[3], lo[3], fv[3];
-#pragma unroll
-                for (int k = 0; k < 3; k++) {
-                    const float h = float(pr.x[k]);
-                    hi[k] = empty ? kFarF : h;
-                    lo[k] = empty ? 0.0f : float(pr.x[k] - double(h));
-                    fv[k] = empty ? 0.0f : float(pr.v[k]);
-                }
-                tile.a[t] = make_float4(hi[0], hi[1], hi[2], p.fm[j]);
-                tile.b[t] = make_float4(lo[0], lo[1], lo[2], fv[0]);
-                tile.c[t] = make_float4(fv[1], fv[2], __int_as_float(p.jid[j]), 0.0f);
-            } else {
-                tile.a[t] = make_float4(p.hx[j], p.hy[j], p.hz[j], p.fm[j]);
-                tile.b[t] = make_float4(p.lx[j], p.ly[j], p.lz[j], p.fvx[j]);
-                tile.c[t] = make_float4(p.fvy[j], p.fvz[j], __int_as_float(p.jid[j]), 0.0f);
-            }
-        }
-        __syncthreads();
-        for (int jb = js; jb < jn; jb += NJS * kFlush) {
-            const int je = min(jn, jb + NJS * kFlush);
-#pragma unroll G6_MX_UNROLL
-            for (int jj = jb; jj < je; jj += NJS) {
-                const float4 a = tile.a[jj], b = tile.b[jj], c = tile.c[jj];
-#pragma unroll
-                for (int q = 0; q < IPL; q++)
-                    pair_mx<NNM>(I[q], A[q], a, b, c, tb + jj, eps2);
-            }
-#pragma unroll
-            for (int q = 0; q < IPL; q++) flush_mx(A[q], D[q]);
-            if (NNM == 2) {
-                // A block minimum below the best so far (strictly: an equal
-                // r^2 keeps the earlier block) moves the best to this block.
-#pragma unroll
-                for (int q = 0; q < IPL; q++) {
-                    const bool imp = A[q].tbest < A[q].best;
-                    A[q].best = imp ? A[q].tbest : A[q].best;
-                    A[q].blo = imp ? tb + jb : A[q].blo;
-                    A[q].bhi = imp ? tb + je : A[q].bhi;
-                    A[q].tbest = __builtin_inff();
-                }
-            }
-        }
-    }
-    if (NNM == 2) {
-        // The slot: the first j of the best block, in the lane's stride,
-        // with exactly that r^2 (recomputed from the same arrays the tiles
-        // were loaded from, so bit-identical).  The wave searches one lane's
-        // block at a time, lane k taking the block's k-th j (a block holds
-        // at most kFlush of a lane's j), so each block is read once,
-        // coalesced, and the first match is the lowest set bit of the ballot.
-        const int lane = threadIdx.x & 63;
-#pragma unroll
-        for (int q = 0; q < IPL; q++) {
-            for (int L = 0; L < 64; L++) {
-                const float best = __shfl(A[q].best, L);
-                if (!(best < __builtin_inff())) continue;  // wave-uniform
-                const int blo = __shfl(A[q].blo, L), bhi = __shfl(A[q].bhi, L);
-                IF ii{};
-                ii.hx = __shfl(I[q].hx, L); ii.hy = __shfl(I[q].hy, L);
-                ii.hz = __shfl(I[q].hz, L); ii.lx = __shfl(I[q].lx, L);
-                ii.ly = __shfl(I[q].ly, L); ii.lz = __shfl(I[q].lz, L);
-                ii.id = __shfl(I[q].id, L);
-                const int j = blo + lane * NJS;
-                bool hit = false;
-                if (j < bhi) {
-                    const float4 a = make_float4(p.hx[j], p.hy[j], p.hz[j], 0.0f);
-                    const float4 b = make_float4(p.lx[j], p.ly[j], p.lz[j], 0.0f);
-                    const float4 c = make_float4(0.0f, 0.0f, __int_as_float(p.jid[j]), 0.0f);
-                    float dx, dy, dz;
-                    hit = pair_r2_mx(ii, a, b, c, dx, dy, dz) == best;
-                }
-                const unsigned long long mask = __ballot(hit);
-                if (lane == L && mask) A[q].addr = blo + (__ffsll((long long)mask) - 1) * NJS;
-            }
-        }
-    }
-
     // Combine the NJS lanes of each i (double-single adds).
 #pragma unroll
     for (int q = 0; q < IPL; q++) {
@@ -781,6 +754,365 @@ __device__ __forceinline__ void force_mx_item(const FParams &p, TileF &tile, int
     }
 }
 
+// One (i-block bx, j-chunk by) work item; the kernel below runs one item
+// per workgroup, kd_* below loops over items whose count lives on the device.
+// LONG (with NN): a lane's j in the item number >= kLongItem, so the
+// nearest neighbour is a running minimum per block of kFlush j with the slot
+// found once at the end; otherwise a compare and two selects per pair, as a
+// final search (up to kFlush dependent loads) would dominate a short item.
+// Both give the first slot with the lane's minimal r^2.
+// PRED: as in force_f64_item.  CHK: as in pair_r2_mx.
+template <int NJS, int IPL, bool NN, bool LONG, bool PRED = false, bool CHK = true>
+__device__ __forceinline__ void force_mx_item(const FParams &p, TileF &tile, int bx, int by,
+                                       int nchunk)
+{
+    constexpr int NNM = NN ? (LONG ? 2 : 1) : 0;
+    constexpr int IBL = kWG / NJS;
+    constexpr int IB = IBL * IPL;
+
+    const int t = threadIdx.x;
+    const int js = t % NJS;
+    const int il = t / NJS;
+    const int ibase = bx * IB;
+
+    IF I[IPL];
+    AccF A[IPL];
+    DSAcc D[IPL];
+#pragma unroll
+    for (int q = 0; q < IPL; q++) {
+        const int i = min(ibase + il + q * IBL, p.ni - 1);
+        const double x = p.ix[i], y = p.iy[i], z = p.iz[i];
+        I[q].hx = float(x); I[q].lx = float(x - double(I[q].hx));
+        I[q].hy = float(y); I[q].ly = float(y - double(I[q].hy));
+        I[q].hz = float(z); I[q].lz = float(z - double(I[q].hz));
+        I[q].vx = float(p.ivx[i]); I[q].vy = float(p.ivy[i]); I[q].vz = float(p.ivz[i]);
+        I[q].id = p.iid[i];
+        A[q] = AccF{0, 0, 0, 0, 0, 0, 0, __builtin_inff(), -1, __builtin_inff(), 0, 0};
+#pragma unroll
+        for (int k = 0; k < 7; k++) { D[q].h[k] = 0.0f; D[q].l[k] = 0.0f; }
+    }
+    const float eps2 = float(p.eps2);
+    const int j0 = by * p.jchunk;
+    const int j1 = min(p.nj, j0 + p.jchunk);
+
+    for (int tb = j0; tb < j1; tb += kTJ) {
+        const int jn = min(kTJ, j1 - tb);
+        __syncthreads();
+        if (t < jn) {
+            const int j = tb + t;
+            if constexpr (PRED) {
+                const Pred pr = predict_base(p, j);
+                const bool empty = p.jid[j] == -1;
+                float hi[3], lo[3], fv[3];
+#pragma unroll
+                for (int k = 0; k < 3; k++) {
+                    const float h = float(pr.x[k]);
+                    hi[k] = empty ? kFarF : h;
+                    lo[k] = empty ? 0.0f : float(pr.x[k] - double(h));
+                    fv[k] = empty ? 0.0f : float(pr.v[k]);
+                }
+                tile.a[t] = make_float4(hi[0], hi[1], hi[2], p.fm[j]);
+                tile.b[t] = make_float4(lo[0], lo[1], lo[2], fv[0]);
+                tile.c[t] = make_float4(fv[1], fv[2], __int_as_float(p.jid[j]), 0.0f);
+            } else {
+                tile.a[t] = make_float4(p.hx[j], p.hy[j], p.hz[j], p.fm[j]);
+                tile.b[t] = make_float4(p.lx[j], p.ly[j], p.lz[j], p.fvx[j]);
+                tile.c[t] = make_float4(p.fvy[j], p.fvz[j], __int_as_float(p.jid[j]), 0.0f);
+            }
+        }
+        __syncthreads();
+        for (int jb = js; jb < jn; jb += NJS * kFlush) {
+            const int je = min(jn, jb + NJS * kFlush);
+#pragma unroll G6_MX_UNROLL
+            for (int jj = jb; jj < je; jj += NJS) {
+                const float4 a = tile.a[jj], b = tile.b[jj], c = tile.c[jj];
+#pragma unroll
+                for (int q = 0; q < IPL; q++)
+                    pair_mx<NNM, CHK>(I[q], A[q], a, b, c, tb + jj, eps2);
+            }
+#pragma unroll
+            for (int q = 0; q < IPL; q++) flush_mx(A[q], D[q]);
+            if (NNM == 2) {
+                // A block minimum below the best so far (strictly: an equal
+                // r^2 keeps the earlier block) moves the best to this block.
+#pragma unroll
+                for (int q = 0; q < IPL; q++) {
+                    const bool imp = A[q].tbest < A[q].best;
+                    A[q].best = imp ? A[q].tbest : A[q].best;
+                    A[q].blo = imp ? tb + jb : A[q].blo;
+                    A[q].bhi = imp ? tb + je : A[q].bhi;
+                    A[q].tbest = __builtin_inff();
+                }
+            }
+        }
+    }
+    if (NNM == 2) {
+        // The slot: the first j of the best block, in the lane's stride,
+        // with exactly that r^2 (recomputed from the same arrays the tiles
+        // were loaded from, so bit-identical).  The wave searches one lane's
+        // block at a time, lane k taking the block's k-th j (a block holds
+        // at most kFlush of a lane's j), so each block is read once,
+        // coalesced, and the first match is the lowest set bit of the ballot.
+        const int lane = threadIdx.x & 63;
+#pragma unroll
+        for (int q = 0; q < IPL; q++) {
+            for (int L = 0; L < 64; L++) {
+                const float best = __shfl(A[q].best, L);
+                if (!(best < __builtin_inff())) continue;  // wave-uniform
+                const int blo = __shfl(A[q].blo, L), bhi = __shfl(A[q].bhi, L);
+                IF ii{};
+                ii.hx = __shfl(I[q].hx, L); ii.hy = __shfl(I[q].hy, L);
+                ii.hz = __shfl(I[q].hz, L); ii.lx = __shfl(I[q].lx, L);
+                ii.ly = __shfl(I[q].ly, L); ii.lz = __shfl(I[q].lz, L);
+                ii.id = __shfl(I[q].id, L);
+                const int j = blo + lane * NJS;
+                bool hit = false;
+                if (j < bhi) {
+                    const float4 a = make_float4(p.hx[j], p.hy[j], p.hz[j], 0.0f);
+                    const float4 b = make_float4(p.lx[j], p.ly[j], p.lz[j], 0.0f);
+                    const float4 c = make_float4(0.0f, 0.0f, __int_as_float(p.jid[j]), 0.0f);
+                    float dx, dy, dz;
+                    hit = pair_r2_mx<CHK>(ii, a, b, c, dx, dy, dz) == best;
+                }
+                const unsigned long long mask = __ballot(hit);
+                if (lane == L && mask) A[q].addr = blo + (__ffsll((long long)mask) - 1) * NJS;
+            }
+        }
+    }
+
+    mx_item_epilogue<NJS, IPL, NN>(p, tile, I, A, D, ibase, il, js, by, nchunk);
+}
+
+// ------------------------------------------------------------------------
+// Mixed path over spatially ordered slots (full passes, g6x_calc_self): the
+// i-block and the j-tiles are runs of slots in Morton order, with the
+// bounding box of each tile's positions in p.tbox.  A tile whose box lies at
+// least 2 R from the i-block's box (R: the i-box's half-diagonal) is "far":
+// it is staged with positions relative to the i-box centre c, rounded once
+// to FP32 (x_j - c in double, then float), and the pair takes
+// dx = (x_j - c) - (x_i - c) in one FP32 subtraction instead of the three of
+// the double-single difference.  Its error is at most 2^-24 (|x_j - c| +
+// |x_i - c|) <= 2^-24 (|dx| + 2 R) <= 2^-23 |dx| per component (|dx| >= 2 R
+// there), the same order as the double-single path's final rounding.  Other
+// tiles take the double-single difference with the id compare.  The slots
+// are distinct (the caller's precondition), so a far tile never holds an i
+// of the block.
+// ------------------------------------------------------------------------
+struct IL {
+    float x, y, z;  // position relative to the i-box centre
+};
+
+template <int NNM>
+__device__ __forceinline__ void pair_far(const IL &L, const IF &I, AccF &a, float4 ja, float4 jb,
+                                         float eps2)
+{
+    const float dx = ja.x - L.x, dy = ja.y - L.y, dz = ja.z - L.z;
+    const float r2 = __builtin_fmaf(dz, dz, __builtin_fmaf(dy, dy, dx * dx));
+    const float s = r2 + eps2;
+    const float y = __builtin_amdgcn_rsqf(s);
+    if (NNM == 2) a.tbest = fminf(a.tbest, r2);
+    const float y2 = y * y;
+    const float mi = ja.w * y;
+    const float mi3 = mi * y2;
+    a.pot -= mi;
+    a.ax = __builtin_fmaf(mi3, dx, a.ax);
+    a.ay = __builtin_fmaf(mi3, dy, a.ay);
+    a.az = __builtin_fmaf(mi3, dz, a.az);
+    const float dvx = jb.x - I.vx, dvy = jb.y - I.vy, dvz = jb.z - I.vz;
+    const float rv = __builtin_fmaf(dz, dvz, __builtin_fmaf(dy, dvy, dx * dvx));
+    const float c = (-3.0f * rv) * y2;
+    a.jx = __builtin_fmaf(mi3, __builtin_fmaf(c, dx, dvx), a.jx);
+    a.jy = __builtin_fmaf(mi3, __builtin_fmaf(c, dy, dvy), a.jy);
+    a.jz = __builtin_fmaf(mi3, __builtin_fmaf(c, dz, dvz), a.jz);
+}
+
+// FP32 position of slot j relative to the centre (cx, cy, cz), as staged.
+__device__ __forceinline__ float3 local_pos(const FParams &p, int j, double cx, double cy,
+                                            double cz)
+{
+    return make_float3(float((double(p.hx[j]) + double(p.lx[j])) - cx),
+                       float((double(p.hy[j]) + double(p.ly[j])) - cy),
+                       float((double(p.hz[j]) + double(p.lz[j])) - cz));
+}
+
+template <int NJS, int IPL, bool NN, bool LONG>
+__device__ __forceinline__ void force_mx_item_sp(const FParams &p, TileF &tile, int bx, int by,
+                                                 int nchunk)
+{
+    constexpr int NNM = NN ? (LONG ? 2 : 1) : 0;
+    constexpr int IBL = kWG / NJS;
+    constexpr int IB = IBL * IPL;
+
+    const int t = threadIdx.x;
+    const int js = t % NJS;
+    const int il = t / NJS;
+    const int ibase = bx * IB;
+
+    // the i-block's box: union of the boxes of the tiles holding its rows
+    float lo[3] = {__builtin_inff(), __builtin_inff(), __builtin_inff()};
+    float hi[3] = {-__builtin_inff(), -__builtin_inff(), -__builtin_inff()};
+    const int last = min(p.ni, ibase + IB) - 1;
+    for (int tt = ibase / kTJ; tt <= last / kTJ; tt++) {
+#pragma unroll
+        for (int k = 0; k < 3; k++) {
+            lo[k] = fminf(lo[k], p.tbox[6 * tt + k]);
+            hi[k] = fmaxf(hi[k], p.tbox[6 * tt + 3 + k]);
+        }
+    }
+    const float cxf = 0.5f * (lo[0] + hi[0]), cyf = 0.5f * (lo[1] + hi[1]),
+                czf = 0.5f * (lo[2] + hi[2]);
+    const double cx = cxf, cy = cyf, cz = czf;
+    float r2box = 0.0f;
+#pragma unroll
+    for (int k = 0; k < 3; k++) {
+        const float h = 0.5f * (hi[k] - lo[k]);
+        r2box = __builtin_fmaf(h, h, r2box);
+    }
+
+    IF I[IPL];
+    IL L[IPL];
+    AccF A[IPL];
+    DSAcc D[IPL];
+    int bfar[IPL];  // NNM 2: the best block was in a far tile
+#pragma unroll
+    for (int q = 0; q < IPL; q++) {
+        const int i = min(ibase + il + q * IBL, p.ni - 1);
+        const double x = p.ix[i], y = p.iy[i], z = p.iz[i];
+        I[q].hx = float(x); I[q].lx = float(x - double(I[q].hx));
+        I[q].hy = float(y); I[q].ly = float(y - double(I[q].hy));
+        I[q].hz = float(z); I[q].lz = float(z - double(I[q].hz));
+        I[q].vx = float(p.ivx[i]); I[q].vy = float(p.ivy[i]); I[q].vz = float(p.ivz[i]);
+        I[q].id = p.iid[i];
+        // i-block rows are the slots' own double-single positions summed,
+        // so this equals local_pos of the same slot
+        L[q].x = float(x - cx); L[q].y = float(y - cy); L[q].z = float(z - cz);
+        A[q] = AccF{0, 0, 0, 0, 0, 0, 0, __builtin_inff(), -1, __builtin_inff(), 0, 0};
+        bfar[q] = 0;
+#pragma unroll
+        for (int k = 0; k < 7; k++) { D[q].h[k] = 0.0f; D[q].l[k] = 0.0f; }
+    }
+    const float eps2 = float(p.eps2);
+    const int j0 = by * p.jchunk;
+    const int j1 = min(p.nj, j0 + p.jchunk);
+
+    for (int tb = j0; tb < j1; tb += kTJ) {
+        const int jn = min(kTJ, j1 - tb);
+        // box distance: workgroup-uniform
+        const int tt = tb / kTJ;
+        float d2 = 0.0f;
+#pragma unroll
+        for (int k = 0; k < 3; k++) {
+            const float g = fmaxf(0.0f, fmaxf(p.tbox[6 * tt + k] - hi[k], lo[k] - p.tbox[6 * tt + 3 + k]));
+            d2 = __builtin_fmaf(g, g, d2);
+        }
+        const bool far = d2 > 0.0f && d2 >= 4.0f * r2box;
+        __syncthreads();
+        if (t < jn) {
+            const int j = tb + t;
+            if (far) {
+                const float3 f = local_pos(p, j, cx, cy, cz);
+                tile.a[t] = make_float4(f.x, f.y, f.z, p.fm[j]);
+                tile.b[t] = make_float4(p.fvx[j], p.fvy[j], p.fvz[j], 0.0f);
+            } else {
+                tile.a[t] = make_float4(p.hx[j], p.hy[j], p.hz[j], p.fm[j]);
+                tile.b[t] = make_float4(p.lx[j], p.ly[j], p.lz[j], p.fvx[j]);
+                tile.c[t] = make_float4(p.fvy[j], p.fvz[j], __int_as_float(p.jid[j]), 0.0f);
+            }
+        }
+        __syncthreads();
+        for (int jb = js; jb < jn; jb += NJS * kFlush) {
+            const int je = min(jn, jb + NJS * kFlush);
+            if (far) {
+#pragma unroll G6_MX_UNROLL
+                for (int jj = jb; jj < je; jj += NJS) {
+                    const float4 a = tile.a[jj], b = tile.b[jj];
+#pragma unroll
+                    for (int q = 0; q < IPL; q++) {
+                        if (NNM == 1) {
+                            // compare on the same r^2 the far pair forms
+                            const float dx = a.x - L[q].x, dy = a.y - L[q].y, dz = a.z - L[q].z;
+                            const float r2 = __builtin_fmaf(dz, dz, __builtin_fmaf(dy, dy, dx * dx));
+                            const bool better = r2 < A[q].best;
+                            A[q].best = better ? r2 : A[q].best;
+                            A[q].addr = better ? tb + jj : A[q].addr;
+                        }
+                        pair_far<NNM == 2 ? 2 : 0>(L[q], I[q], A[q], a, b, eps2);
+                    }
+                }
+            } else {
+#pragma unroll G6_MX_UNROLL
+                for (int jj = jb; jj < je; jj += NJS) {
+                    const float4 a = tile.a[jj], b = tile.b[jj], c = tile.c[jj];
+#pragma unroll
+                    for (int q = 0; q < IPL; q++)
+                        pair_mx<NNM, true>(I[q], A[q], a, b, c, tb + jj, eps2);
+                }
+            }
+#pragma unroll
+            for (int q = 0; q < IPL; q++) flush_mx(A[q], D[q]);
+            if (NNM == 2) {
+#pragma unroll
+                for (int q = 0; q < IPL; q++) {
+                    const bool imp = A[q].tbest < A[q].best;
+                    A[q].best = imp ? A[q].tbest : A[q].best;
+                    A[q].blo = imp ? tb + jb : A[q].blo;
+                    A[q].bhi = imp ? tb + je : A[q].bhi;
+                    bfar[q] = imp ? int(far) : bfar[q];
+                    A[q].tbest = __builtin_inff();
+                }
+            }
+        }
+    }
+    if (NNM == 2) {
+        // the slot, as in force_mx_item, with r^2 recomputed the way its
+        // tile formed it (relative to c for a far tile)
+        const int lane = threadIdx.x & 63;
+#pragma unroll
+        for (int q = 0; q < IPL; q++) {
+            for (int Ln = 0; Ln < 64; Ln++) {
+                const float best = __shfl(A[q].best, Ln);
+                if (!(best < __builtin_inff())) continue;  // wave-uniform
+                const int blo = __shfl(A[q].blo, Ln), bhi = __shfl(A[q].bhi, Ln);
+                const int isfar = __shfl(bfar[q], Ln);
+                const int j = blo + lane * NJS;
+                bool hit = false;
+                if (isfar) {
+                    const float lx = __shfl(L[q].x, Ln), ly = __shfl(L[q].y, Ln),
+                                lz = __shfl(L[q].z, Ln);
+                    if (j < bhi) {
+                        const float3 f = local_pos(p, j, cx, cy, cz);
+                        const float dx = f.x - lx, dy = f.y - ly, dz = f.z - lz;
+                        hit = __builtin_fmaf(dz, dz, __builtin_fmaf(dy, dy, dx * dx)) == best;
+                    }
+                } else {
+                    IF ii{};
+                    ii.hx = __shfl(I[q].hx, Ln); ii.hy = __shfl(I[q].hy, Ln);
+                    ii.hz = __shfl(I[q].hz, Ln); ii.lx = __shfl(I[q].lx, Ln);
+                    ii.ly = __shfl(I[q].ly, Ln); ii.lz = __shfl(I[q].lz, Ln);
+                    ii.id = __shfl(I[q].id, Ln);
+                    if (j < bhi) {
+                        const float4 a = make_float4(p.hx[j], p.hy[j], p.hz[j], 0.0f);
+                        const float4 b = make_float4(p.lx[j], p.ly[j], p.lz[j], 0.0f);
+                        const float4 c = make_float4(0.0f, 0.0f, __int_as_float(p.jid[j]), 0.0f);
+                        float dx, dy, dz;
+                        hit = pair_r2_mx<true>(ii, a, b, c, dx, dy, dz) == best;
+                    }
+                }
+                const unsigned long long mask = __ballot(hit);
+                if (lane == Ln && mask) A[q].addr = blo + (__ffsll((long long)mask) - 1) * NJS;
+            }
+        }
+    }
+    mx_item_epilogue<NJS, IPL, NN>(p, tile, I, A, D, ibase, il, js, by, nchunk);
+}
+
+template <int NJS, int IPL, bool NN, bool LONG>
+__global__ void __launch_bounds__(kWG) k_force_mx_sp(FParams p)
+{
+    __shared__ TileF tile;
+    force_mx_item_sp<NJS, IPL, NN, LONG>(p, tile, blockIdx.x, blockIdx.y, gridDim.y);
+}
+
 template <bool NN>
 __global__ void __launch_bounds__(kWG) k_force_mx_p(FParams p)
 {
@@ -788,14 +1120,22 @@ __global__ void __launch_bounds__(kWG) k_force_mx_p(FParams p)
     force_mx_item<64, 1, NN, false, true>(p, tile, blockIdx.x, 0, 1);
 }
 
-template <int NJS, int IPL, bool NN>
+template <int NJS, int IPL, bool NN, bool LONG>
 __global__ void __launch_bounds__(kWG) k_force_mx(FParams p)
 {
     __shared__ TileF tile;
-    if (NN && (p.jchunk + NJS - 1) / NJS >= kLongItem)
-        force_mx_item<NJS, IPL, NN, true>(p, tile, blockIdx.x, blockIdx.y, gridDim.y);
+    if (needs_id_check(p, blockIdx.x * (kWG / NJS) * IPL, (kWG / NJS) * IPL, blockIdx.y))
+        force_mx_item<NJS, IPL, NN, LONG>(p, tile, blockIdx.x, blockIdx.y, gridDim.y);
     else
-        force_mx_item<NJS, IPL, NN, false>(p, tile, blockIdx.x, blockIdx.y, gridDim.y);
+        force_mx_item<NJS, IPL, NN, LONG, false, false>(p, tile, blockIdx.x, blockIdx.y,
+                                                        gridDim.y);
+}
+
+// Items whose lanes see >= kLongItem j keep the nearest neighbour as a
+// running minimum (the LONG forms).
+__host__ __device__ inline bool long_items(const FParams &p, int njs, bool nn)
+{
+    return nn && (p.jchunk + njs - 1) / njs >= kLongItem;
 }
 
 // ------------------------------------------------------------------------
@@ -820,7 +1160,7 @@ __global__ void __launch_bounds__(kWG) k_force_f64_s(FParams p)
         I[q].x = p.ix[i]; I[q].y = p.iy[i]; I[q].z = p.iz[i];
         I[q].vx = p.ivx[i]; I[q].vy = p.ivy[i]; I[q].vz = p.ivz[i];
         I[q].id = p.iid[i];
-        A[q] = AccD{0, 0, 0, 0, 0, 0, 0, __builtin_inf(), -1};
+        A[q] = AccD{0, 0, 0, 0, 0, 0, 0, __builtin_inf(), -1, __builtin_inf(), 0, 0};
 #pragma unroll
         for (int k = 0; k < 7; k++) { D[q].h[k] = 0.0; D[q].l[k] = 0.0; }
     }
@@ -1241,21 +1581,42 @@ static void launch_force_s(const FParams &fp, int prec, bool nn, dim3 grid, hipS
 template <int NJS, int IPL>
 static void launch_force_t(const FParams &fp, int prec, bool nn, dim3 grid, hipStream_t st)
 {
+    const bool lng = long_items(fp, NJS, nn);
     if (prec == kF64) {
-        if (nn) hipLaunchKernelGGL((k_force_f64<NJS, IPL, true>), grid, dim3(kWG), 0, st, fp);
-        else hipLaunchKernelGGL((k_force_f64<NJS, IPL, false>), grid, dim3(kWG), 0, st, fp);
+        if (lng) hipLaunchKernelGGL((k_force_f64<NJS, IPL, true, true>), grid, dim3(kWG), 0, st, fp);
+        else if (nn) hipLaunchKernelGGL((k_force_f64<NJS, IPL, true, false>), grid, dim3(kWG), 0, st, fp);
+        else hipLaunchKernelGGL((k_force_f64<NJS, IPL, false, false>), grid, dim3(kWG), 0, st, fp);
     } else {
-        if (nn) hipLaunchKernelGGL((k_force_mx<NJS, IPL, true>), grid, dim3(kWG), 0, st, fp);
-        else hipLaunchKernelGGL((k_force_mx<NJS, IPL, false>), grid, dim3(kWG), 0, st, fp);
+        if (lng) hipLaunchKernelGGL((k_force_mx<NJS, IPL, true, true>), grid, dim3(kWG), 0, st, fp);
+        else if (nn) hipLaunchKernelGGL((k_force_mx<NJS, IPL, true, false>), grid, dim3(kWG), 0, st, fp);
+        else hipLaunchKernelGGL((k_force_mx<NJS, IPL, false, false>), grid, dim3(kWG), 0, st, fp);
     }
+}
+
+bool spatial_cfg_supported(const ForceCfg &c)
+{
+    return c.njs == 1 && (c.ipl == 1 || c.ipl == 2) && !c.smem;
+}
+
+template <int IPL>
+static void launch_force_sp(const FParams &fp, bool nn, dim3 grid, hipStream_t st)
+{
+    if (long_items(fp, 1, nn))
+        hipLaunchKernelGGL((k_force_mx_sp<1, IPL, true, true>), grid, dim3(kWG), 0, st, fp);
+    else if (nn)
+        hipLaunchKernelGGL((k_force_mx_sp<1, IPL, true, false>), grid, dim3(kWG), 0, st, fp);
+    else
+        hipLaunchKernelGGL((k_force_mx_sp<1, IPL, false, false>), grid, dim3(kWG), 0, st, fp);
 }
 
 hipError_t launch_force(const JStore &s, const IBlock &b, int ni, int nj, double eps2, int prec,
                         bool want_nn, const ForceCfg &cfg, hipStream_t st, bool pred_j,
-                        double ti, int compat)
+                        double ti, int compat, int self_i0, const float *tbox)
 {
     if (ni <= 0) return hipSuccess;
     FParams fp{};
+    fp.self_i0 = pred_j ? -1 : self_i0;
+    fp.tbox = tbox;
     fp.ix = b.x[0]; fp.iy = b.x[1]; fp.iz = b.x[2];
     fp.ivx = b.v[0]; fp.ivy = b.v[1]; fp.ivz = b.v[2];
     fp.iid = b.id;
@@ -1291,6 +1652,13 @@ hipError_t launch_force(const JStore &s, const IBlock &b, int ni, int nj, double
             if (want_nn) hipLaunchKernelGGL((k_force_mx_p<true>), grid, dim3(kWG), 0, st, fp);
             else hipLaunchKernelGGL((k_force_mx_p<false>), grid, dim3(kWG), 0, st, fp);
         }
+        return hipGetLastError();
+    }
+    if (tbox) {
+        // spatially ordered full pass, mixed path (the caller checked the shape)
+        if (prec != kMixed || !spatial_cfg_supported(cfg)) return hipErrorInvalidValue;
+        if (cfg.ipl == 1) launch_force_sp<1>(fp, want_nn, grid, st);
+        else launch_force_sp<2>(fp, want_nn, grid, st);
         return hipGetLastError();
     }
     if (cfg.smem) {

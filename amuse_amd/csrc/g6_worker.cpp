@@ -28,9 +28,14 @@
 //
 // Functions: ph4's interface (src/amuse_ph4/interface.cc, interface.py,
 // src/amuse/community/interface/gd.py) with the same names, parameters and
-// return codes, backed by the device-resident driver.  Particles are added
-// before commit_particles; adding or deleting after it returns -2 (the
-// driver holds the set on the GPU).
+// return codes, backed by the device-resident driver.  After
+// commit_particles, synchronize_model steps every particle to the system
+// time (ph4's synchronize_all); particles added, deleted or changed
+// (new_particle, delete_particle, set_state/mass/position/velocity) edit the
+// host copy of the state, and recommit_particles (or the next evolve_model)
+// rebuilds the driver from it at the system time: forces and initial steps
+// recomputed, as ph4's recommit_particles does (interface.cc:349-379), with
+// the model time and the step counters carried over.
 #include "../../include/g6x.h"
 
 #include <arpa/inet.h>
@@ -235,6 +240,11 @@ struct Code {
     void *drv = nullptr;
     bool state_fresh = false;
     std::vector<double> t, dt;
+    // a driver rebuilt by recommit starts at system time 0: the model time
+    // is begin_time + time_offset + its system time, the step counters add
+    bool dirty = false;  // the host copy was edited after commit
+    double time_offset = 0.0;
+    long long blocks_offset = 0, steps_offset = 0;
 
     void reset()
     {
@@ -243,6 +253,51 @@ struct Code {
         id.clear(); mass.clear(); radius.clear(); pos.clear(); vel.clear(); slot.clear();
         next_id = 1;
         state_fresh = false;
+        dirty = false;
+        time_offset = 0.0;
+        blocks_offset = steps_offset = 0;
+    }
+    void reindex()
+    {
+        slot.clear();
+        for (size_t q = 0; q < id.size(); q++) slot[id[q]] = int(q);
+    }
+    int counters(long long &b, long long &p, double &ts)
+    {
+        b = p = 0;
+        ts = 0.0;
+        if (drv && g6d_counters(drv, &b, &p, &ts) < 0) return -1;
+        b += blocks_offset;
+        p += steps_offset;
+        ts += time_offset;
+        return 0;
+    }
+    int create()
+    {
+        const int n = int(id.size());
+        drv = g6d_create(n, id.data(), mass.data(),
+                         reinterpret_cast<const double(*)[3]>(pos.data()),
+                         reinterpret_cast<const double(*)[3]>(vel.data()), eps2, eta, dt_fac,
+                         dt_limit, dt_median, precision, 1, gpu_id < 0 ? 0 : gpu_id);
+        state_fresh = false;
+        dirty = false;
+        return drv ? 0 : -1;
+    }
+    // The driver rebuilt from the host copy at the system time.
+    int recommit()
+    {
+        if (!drv) return -1;
+        if (refresh()) return -1;
+        long long b = 0, p = 0;
+        double ts = 0.0;
+        if (g6d_counters(drv, &b, &p, &ts) < 0) return -1;
+        g6d_free(drv);
+        drv = nullptr;
+        blocks_offset += b;
+        steps_offset += p;
+        time_offset += ts;
+        if (id.empty()) return 0;
+        return create();
     }
     int index_of(int pid) const
     {
@@ -335,13 +390,20 @@ std::vector<Function> functions()
     // new_particle(out index, mass, x, y, z, vx, vy, vz, radius, id): ids
     // from 1 up unless a free id >= 0 is asked for (jdata.cc:190-230)
     f.push_back({"new_particle", 1, 8, 1, 0, [](Call &c) {
-                     if (code.drv) return -2;
+                     if (code.drv) {  // the new body joins at the next recommit
+                         if (code.refresh()) return -1;
+                         code.dirty = true;
+                     }
                      int pid = c.ii(0);
                      if (pid < 0 || code.slot.count(pid)) {
                          while (code.slot.count(code.next_id)) code.next_id++;
                          pid = code.next_id++;
                      }
                      code.slot[pid] = int(code.id.size());
+                     if (code.drv) {  // time and step are set by the recommit
+                         code.t.push_back(0.0);
+                         code.dt.push_back(0.0);
+                     }
                      code.id.push_back(pid);
                      code.mass.push_back(c.di(0));
                      for (int k = 0; k < 3; k++) code.pos.push_back(c.di(1 + k));
@@ -351,41 +413,77 @@ std::vector<Function> functions()
                      return 0;
                  }});
     f.push_back({"delete_particle", 1, 0, 0, 0, [](Call &c) {
-                     if (code.drv) return -2;
                      const int j = code.index_of(c.ii(0));
                      if (j < 0) return -1;
+                     if (code.drv) {
+                         if (code.refresh()) return -1;
+                         code.dirty = true;
+                         code.t.erase(code.t.begin() + j);
+                         code.dt.erase(code.dt.begin() + j);
+                     }
                      code.id.erase(code.id.begin() + j);
                      code.mass.erase(code.mass.begin() + j);
                      code.radius.erase(code.radius.begin() + j);
                      code.pos.erase(code.pos.begin() + 3 * j, code.pos.begin() + 3 * j + 3);
                      code.vel.erase(code.vel.begin() + 3 * j, code.vel.begin() + 3 * j + 3);
-                     code.slot.clear();
-                     for (size_t q = 0; q < code.id.size(); q++) code.slot[code.id[q]] = int(q);
+                     code.reindex();
                      return 0;
                  }});
+    // setters (interface.cc:466-580): ph4 writes its arrays; here the host
+    // copy, which the next recommit loads (radius alone does not enter the
+    // dynamics)
+    auto edit = [&](const char *name, int nd, int what) {
+        f.push_back({name, 1, nd, 0, 0, [what](Call &c) {
+                         const int j = code.index_of(c.ii(0));
+                         if (j < 0) return -1;
+                         if (code.drv && code.refresh()) return -1;
+                         int k = 0;
+                         if (what == 0 || what == 1) code.mass[j] = c.di(k++);
+                         if (what == 0 || what == 3)
+                             for (int q = 0; q < 3; q++) code.pos[3 * j + q] = c.di(k++);
+                         if (what == 0 || what == 4)
+                             for (int q = 0; q < 3; q++) code.vel[3 * j + q] = c.di(k++);
+                         if (what == 0 || what == 2) code.radius[j] = c.di(k++);
+                         if (code.drv && what != 2) code.dirty = true;
+                         return 0;
+                     }});
+    };
+    edit("set_state", 8, 0);
+    edit("set_mass", 1, 1);
+    edit("set_radius", 1, 2);
+    edit("set_position", 3, 3);
+    edit("set_velocity", 3, 4);
     f.push_back({"commit_particles", 0, 0, 0, 0, [](Call &) {
                      if (code.drv || code.id.empty()) return -1;
-                     const int n = int(code.id.size());
-                     code.drv = g6d_create(n, code.id.data(), code.mass.data(),
-                                           reinterpret_cast<const double(*)[3]>(code.pos.data()),
-                                           reinterpret_cast<const double(*)[3]>(code.vel.data()),
-                                           code.eps2, code.eta, code.dt_fac, code.dt_limit,
-                                           code.dt_median, code.precision, 1,
-                                           code.gpu_id < 0 ? 0 : code.gpu_id);
-                     if (!code.drv) {
+                     code.time_offset = 0.0;
+                     code.blocks_offset = code.steps_offset = 0;
+                     if (code.create()) {
                          fprintf(stderr, "g6_worker: commit_particles: no GPU or a HIP error\n");
                          return -1;
                      }
-                     code.state_fresh = false;
                      return 0;
                  }});
-    f.push_back({"recommit_particles", 0, 0, 0, 0, [](Call &) { return code.drv ? 0 : -1; }});
-    f.push_back({"synchronize_model", 0, 0, 0, 0, [](Call &) { return 0; }});
+    f.push_back({"recommit_particles", 0, 0, 0, 0, [](Call &) {
+                     if (code.recommit()) {
+                         fprintf(stderr, "g6_worker: recommit_particles: not committed, no GPU "
+                                         "or a HIP error\n");
+                         return -1;
+                     }
+                     return 0;
+                 }});
+    f.push_back({"synchronize_model", 0, 0, 0, 0, [](Call &) {
+                     if (!code.drv) return 0;
+                     if (code.dirty && code.recommit()) return -1;
+                     code.state_fresh = false;
+                     return g6d_synchronize(code.drv) < 0 ? -1 : 0;
+                 }});
     f.push_back({"evolve_model", 0, 1, 0, 0, [](Call &c) {
                      if (not_committed()) return -1;
+                     if (code.dirty && code.recommit()) return -1;
                      int reached = 0;
-                     const long long r = g6d_run(code.drv, c.di(0) - code.begin_time, 0, 64,
-                                                 &reached);
+                     const long long r =
+                         g6d_run(code.drv, c.di(0) - code.begin_time - code.time_offset, 0, 64,
+                                 &reached);
                      code.state_fresh = false;
                      if (r < 0) {
                          fprintf(stderr, "g6_worker: evolve_model: %s\n",
@@ -397,7 +495,7 @@ std::vector<Function> functions()
     f.push_back({"get_time", 0, 0, 0, 1, [](Call &c) {
                      long long b = 0, p = 0;
                      double ts = 0.0;
-                     if (code.drv && g6d_counters(code.drv, &b, &p, &ts) < 0) return -1;
+                     if (code.counters(b, p, ts)) return -1;
                      c.dout(0) = ts + code.begin_time;
                      return 0;
                  }});
@@ -405,7 +503,7 @@ std::vector<Function> functions()
         f.push_back({name, 0, 0, 1, 0, [blocks](Call &c) {
                          long long b = 0, p = 0;
                          double ts = 0.0;
-                         if (code.drv && g6d_counters(code.drv, &b, &p, &ts) < 0) return -1;
+                         if (code.counters(b, p, ts)) return -1;
                          c.io(0) = int(blocks ? b : p);
                          return 0;
                      }});
@@ -425,6 +523,7 @@ std::vector<Function> functions()
     auto energy = [&](const char *name, bool kinetic) {
         f.push_back({name, 0, 0, 0, 1, [kinetic](Call &c) {
                          if (not_committed()) return -1;
+                         if (code.dirty && code.recommit()) return -1;
                          double k = 0.0, u = 0.0;
                          if (g6d_energy(code.drv, &k, &u) < 0) return -1;
                          c.dout(0) = kinetic ? k : u;
@@ -469,7 +568,9 @@ std::vector<Function> functions()
     vec3("get_position", true);
     vec3("get_velocity", false);
     f.push_back({"get_particle_timestep", 1, 0, 0, 1, [](Call &c) {
-                     if (not_committed() || code.refresh()) return -1;
+                     if (not_committed()) return -1;
+                     if (code.dirty && code.recommit()) return -1;
+                     if (code.refresh()) return -1;
                      const int j = code.index_of(c.ii(0));
                      if (j < 0) return -1;
                      c.dout(0) = code.dt[j];

@@ -18,7 +18,11 @@
 * :func:`lagrangian_radii` and :func:`radii_loss` — mass radii about the
   centre of mass (the quantity of src/amuse/datamodel/particle_attributes.py
   LagrangianRadii, here about the COM), linearly interpolated in the
-  cumulative mass so they depend smoothly on masses and positions.
+  cumulative mass so they depend smoothly on masses and positions.  The
+  nodes sit at the middle of each particle's cumulative-mass interval: with
+  equal masses the fractions 0.1 / 0.5 / 0.9 would otherwise fall exactly on
+  a node for many N (0.5 for every even N), a kink where the gradient in the
+  masses is one-sided.
 """
 from __future__ import annotations
 
@@ -216,13 +220,16 @@ def hermite_integrate(x0, v0, m, dt: float, nsteps: int, eps2: float, checkpoint
 
 
 def lagrangian_radii(x, m, fractions=(0.1, 0.5, 0.9)):
-    """Mass radii about the centre of mass, interpolated in cumulative mass."""
+    """Mass radii about the centre of mass, interpolated in cumulative mass
+    between nodes at the middle of each particle's cumulative-mass
+    interval."""
     M = m.sum()
     com = (m[:, None] * x).sum(0) / M
     r = torch.linalg.vector_norm(x - com, dim=1)
     order = torch.argsort(r.detach())
     rs = r[order]
-    cm = torch.cumsum(m[order], 0) / M
+    ms = m[order]
+    cm = (torch.cumsum(ms, 0) - 0.5 * ms) / M
     out = []
     for q in fractions:
         qt = torch.as_tensor(q, dtype=x.dtype, device=x.device)

@@ -4,8 +4,9 @@ Each cluster is integrated with the shared-step 4th-order Hermite scheme of
 src/amuse_hermite/interface.cc (predictor :266-281, collision-time estimate
 :496-531, corrector :536-556, step = dt_param x collision time :618-637)
 with the step rounded down to a power of two dividing the cluster time; all
-clusters advance together (one persistent force launch per step, steps
-replayed from a hipGraph).
+clusters advance together: the whole run in one persistent cooperative
+launch (run(persistent=True)), or one persistent force launch per step with
+the steps replayed from a hipGraph.
 """
 from __future__ import annotations
 
@@ -29,6 +30,8 @@ def _bind(lib):
         "g6c_forces": (C.c_int, [C.c_void_p, C.c_int, _D, _D, _D]),
         "g6c_set_tiles": (C.c_int, [C.c_void_p, _I]),
         "g6c_get_tiles": (C.c_int, [C.c_void_p, _I]),
+        "g6c_set_jtiles": (C.c_int, [C.c_void_p, _I]),
+        "g6c_get_jtiles": (C.c_int, [C.c_void_p, _I]),
         "g6c_autotune": (C.c_int, [C.c_void_p]),
         "g6c_set_precision": (C.c_int, [C.c_void_p, C.c_int]),
         "g6c_elapsed_ms": (C.c_double, [C.c_void_p]),
@@ -68,8 +71,15 @@ class Ensemble:
     def start(self, t_end: float) -> None:
         self._check(self.lib.g6c_start(self.h, float(t_end)), "g6c_start")
 
-    def run(self, steps_per_graph: int = 8, use_graph: bool = True, max_rounds: int = 0) -> int:
-        return self._check(self.lib.g6c_run(self.h, int(steps_per_graph), int(bool(use_graph)),
+    def run(self, steps_per_graph: int = 8, use_graph: bool = True, max_rounds: int = 0,
+            persistent: bool = False) -> int:
+        """Step rounds until every cluster reached t_end (or max_rounds > 0
+        rounds).  persistent: the whole run in one cooperative launch (the
+        four phases of a round separated by grid barriers, the work list
+        rebuilt on the device); otherwise steps_per_graph steps per hipGraph
+        replay (use_graph) or per eager batch."""
+        mode = 2 if persistent else int(bool(use_graph))
+        return self._check(self.lib.g6c_run(self.h, int(steps_per_graph), mode,
                                             int(max_rounds)), "g6c_run")
 
     def energy(self) -> np.ndarray:
@@ -101,6 +111,15 @@ class Ensemble:
     def tiles(self) -> list:
         arr = np.zeros(8, dtype=np.int32)
         self.lib.g6c_get_tiles(self.h, arr.ctypes.data_as(_I))
+        return arr.tolist()
+
+    def set_jtiles(self, tj_per_class) -> None:
+        arr = np.ascontiguousarray(tj_per_class, dtype=np.int32)
+        self._check(self.lib.g6c_set_jtiles(self.h, arr.ctypes.data_as(_I)), "g6c_set_jtiles")
+
+    def jtiles(self) -> list:
+        arr = np.zeros(8, dtype=np.int32)
+        self.lib.g6c_get_jtiles(self.h, arr.ctypes.data_as(_I))
         return arr.tolist()
 
     def autotune(self) -> int:

@@ -45,7 +45,7 @@ EXT_SYMBOLS = [
     "g6x_set_precision", "g6x_get_precision", "g6x_set_predictor_compat", "g6x_set_npipes",
     "g6x_last_error", "g6x_set_j_particles", "g6x_calc_self", "g6x_fetch", "g6x_sync",
     "g6x_timing", "g6x_get_timing", "g6x_set_tiles", "g6x_autotune", "g6x_save_tuning",
-    "g6x_load_tuning",
+    "g6x_load_tuning", "g6x_set_spatial",
 ]
 
 
@@ -84,6 +84,7 @@ def load_library(path: str = LIBPATH) -> C.CDLL:
         "g6x_calc_self": (C.c_int, [C.c_int, C.c_int, C.c_int, C.c_double, C.c_int]),
         "g6x_fetch": (C.c_int, [C.c_int, _D, _D, _D, _I]),
         "g6x_sync": (C.c_int, []),
+        "g6x_set_spatial": (C.c_int, [C.c_int]),
         "g6x_timing": (C.c_int, [C.c_int]),
         "g6x_get_timing": (C.c_int, [_D, _D, _D, C.POINTER(C.c_longlong), _D]),
         "g6x_set_tiles": (C.c_int, [C.c_int, C.c_int, C.c_int, C.c_int]),
@@ -249,6 +250,9 @@ class G6:
         self._check(self.lib.g6x_fetch(int(ni), _d(acc), _d(jerk), _d(pot),
                                        _i(nnb) if nn else None), "g6x_fetch")
         return acc, jerk, pot, nnb
+
+    def set_spatial(self, on: bool) -> None:
+        self._check(self.lib.g6x_set_spatial(int(bool(on))), "g6x_set_spatial")
 
     def sync(self) -> None:
         self._check(self.lib.g6x_sync(), "g6x_sync")

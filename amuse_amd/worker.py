@@ -62,6 +62,12 @@ SPECS = {
     "new_particle": (["id"], ["mass", "x", "y", "z", "vx", "vy", "vz", "radius"],
                      ["index_of_the_particle"], []),
     "delete_particle": (["index_of_the_particle"], [], [], []),
+    "set_state": (["index_of_the_particle"], ["mass", "x", "y", "z", "vx", "vy", "vz", "radius"],
+                  [], []),
+    "set_mass": (["index_of_the_particle"], ["mass"], [], []),
+    "set_radius": (["index_of_the_particle"], ["radius"], [], []),
+    "set_position": (["index_of_the_particle"], ["x", "y", "z"], [], []),
+    "set_velocity": (["index_of_the_particle"], ["vx", "vy", "vz"], [], []),
     "commit_particles": ([], [], [], []),
     "recommit_particles": ([], [], [], []),
     "synchronize_model": ([], [], [], []),

@@ -487,8 +487,10 @@ def worker_study(args) -> int:
 def clusters_c2(args) -> int:
     """Configuration c2: 4096 clusters, N log-uniform in [16, 2048], Kroupa
     0.08-20 normalised to M = 1 per cluster, eps2 = 1e-6, all to t = 1 in
-    mixed precision with autotuned tiles and hipGraph-captured steps; the
-    same ensemble in FP64 for the energy comparison."""
+    mixed precision with autotuned tiles (lanes per i and j per tile), the
+    whole run in one persistent cooperative launch (--c2-mode persistent) or
+    as hipGraph-captured steps (--c2-mode graph); the same ensemble in FP64
+    for the energy comparison."""
     from amuse_amd.ensemble import Ensemble, make_ensemble
     nc = 4096 if args.clusters <= 0 else args.clusters
     counts, m, x, v = make_ensemble(nc, 2048)
@@ -498,18 +500,24 @@ def clusters_c2(args) -> int:
         ens = Ensemble(counts, m, x, v, eps2=1e-6, dt_param=args.dt_param_c2, precision=prec)
         tuned = ens.autotune()
         tiles = ens.tiles()
+        jtiles = ens.jtiles()
         e0 = ens.energy()
         ens.start(1.0)
         limit = args.max_rounds if args.max_rounds > 0 else 4000000
-        chunk = 4096  # step rounds per call, so progress is printed
         t0 = time.perf_counter()
         rounds = 0
-        while rounds < limit:
-            r = ens.run(steps_per_graph=16, use_graph=True, max_rounds=min(chunk, limit - rounds))
-            rounds += r
-            log(f"  {name}: {rounds} step rounds, {time.perf_counter() - t0:.1f} s")
-            if r < chunk:
-                break
+        if args.c2_mode == "persistent":
+            log(f"  {name}: one persistent launch")
+            rounds = ens.run(persistent=True, max_rounds=limit)
+        else:
+            chunk = 4096  # step rounds per call, so progress is printed
+            while rounds < limit:
+                r = ens.run(steps_per_graph=16, use_graph=True,
+                            max_rounds=min(chunk, limit - rounds))
+                rounds += r
+                log(f"  {name}: {rounds} step rounds, {time.perf_counter() - t0:.1f} s")
+                if r < chunk:
+                    break
         wall = time.perf_counter() - t0
         e1 = ens.energy()
         _, _, tc, ns = ens.state()
@@ -519,6 +527,7 @@ def clusters_c2(args) -> int:
         res[name] = {"wall_s": wall, "step_rounds": int(rounds),
                      "cluster_steps_total": int(ns.sum()), "finished": int((tc >= 1.0).sum()),
                      "gpairs_per_s": pairs / wall / 1e9, "tiles_njs_per_size_class": tiles,
+                     "tiles_j_per_size_class": jtiles, "mode": args.c2_mode,
                      "autotune_timings": int(tuned),
                      "dE_over_E_median": float(np.median(de)),
                      "dE_over_E_p90": float(np.quantile(de, 0.9)),
@@ -621,6 +630,8 @@ def main() -> int:
     ap.add_argument("--cpu-threads", type=int, default=0)
     ap.add_argument("--cpu-per-thread", type=int, default=256)
     ap.add_argument("--clusters", type=int, default=0, help="c2: number of clusters")
+    ap.add_argument("--c2-mode", choices=["persistent", "graph"], default="persistent",
+                    help="c2: one persistent launch for the run, or graph-replayed steps")
     ap.add_argument("--dt-param-c2", type=float, default=0.03,
                     help="c2: hermite dt_param (step = dt_param x collision time)")
     ap.add_argument("--max-rounds", type=int, default=0)

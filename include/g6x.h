@@ -63,8 +63,16 @@ int g6x_set_j_particles(int n, const int address[], const int index[],
 int g6x_calc_self(int nj, int i0, int ni, double eps2, int want_nn);
 int g6x_fetch(int ni, double acc[][3], double jerk[][3], double pot[],
               int nn[]);
+
 /* Waits for all work of the unit. */
 int g6x_sync(void);
+/* Full passes in spatial order (default 1): a g6x_calc_self over every slot
+ * (i0 = 0, ni = nj >= 131072) in the mixed path, with every slot holding a
+ * distinct id, sorts the slots along a Morton curve of their positions and
+ * takes pairs of far-apart i-blocks and j-tiles with one FP32 subtraction
+ * per component relative to the i-block's centre (same error bound as the
+ * double-single difference).  0: slot order always. */
+int g6x_set_spatial(int on);
 
 /* Kernel timing with HIP events recorded on the unit's own stream around
  * every force, reduce and predictor launch.  g6x_timing(1) resets and
@@ -189,7 +197,9 @@ int g6a_step_predict_t_f64(int n, const double *gxp, const double *gvp, double d
  * g6d_state: x and v predicted to the system time.  g6d_state_raw: x and v
  * as last corrected, at each particle's own time (ph4's get_state,
  * src/amuse_ph4/interface.cc:485-502).  g6d_steps: per-particle
- * time and step.  g6d_counters: block steps, particle steps, system time. */
+ * time and step.  g6d_counters: block steps, particle steps, system time.
+ * g6d_synchronize: every particle behind the system time steps to it (ph4's
+ * synchronize_all, src/amuse_ph4/src/jdata.cc:874-900), one block step. */
 void *g6d_create(int n, const int ids[], const double m[], const double x[][3],
                  const double v[][3], double eps2, double eta, double dt_fac,
                  double dt_limit, double dt_limitm, int precision,
@@ -203,6 +213,7 @@ int g6d_steps(void *drv, double time[], double step[]);
 int g6d_counters(void *drv, long long *block_steps, long long *particle_steps,
                  double *system_time);
 int g6d_set_precision(void *drv, int precision);
+int g6d_synchronize(void *drv);
 const char *g6d_last_error(void *drv);
 void g6d_free(void *drv);
 

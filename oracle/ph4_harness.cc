@@ -13,15 +13,18 @@
 //   when dtmin is not NULL, the smallest particle time step.
 //
 // Encounter management is off (manage_encounters = 0) so only the Hermite
-// block-step path runs.
+// block-step path runs.  ph4h_run_sync also synchronises every particle to
+// the system time after each output (AMUSE's synchronize_model,
+// src/amuse_ph4/interface.cc:713-724, i.e. jdata::synchronize_all), before
+// the energies are taken.
 #include "idata.h"
 #include "jdata.h"
 #include "scheduler.h"
 
 extern "C" __attribute__((visibility("default"))) int
-ph4h_run(int n, const double *m, const double *x, const double *v, double eps2, double eta,
-         int nout, const double *t_out, double *kin, double *pot, long long *blocks,
-         long long *steps, double *dtmin)
+ph4h_run_sync(int n, const double *m, const double *x, const double *v, double eps2,
+              double eta, int nout, const double *t_out, double *kin, double *pot,
+              long long *blocks, long long *steps, double *dtmin, int sync)
 {
     jdata *jd = new jdata;
     jd->setup_gpu();
@@ -40,6 +43,7 @@ ph4h_run(int n, const double *m, const double *x, const double *v, double eps2, 
     scheduler *s = new scheduler(jd);
     for (int k = 0; k < nout; k++) {
         while (jd->system_time < t_out[k]) jd->advance();
+        if (sync) jd->synchronize_all();
         kin[k] = jd->get_kin();
         pot[k] = jd->get_pot();
         blocks[k] = jd->block_steps;
@@ -55,4 +59,12 @@ ph4h_run(int n, const double *m, const double *x, const double *v, double eps2, 
     delete id;
     delete jd;
     return used_library;
+}
+
+extern "C" __attribute__((visibility("default"))) int
+ph4h_run(int n, const double *m, const double *x, const double *v, double eps2, double eta,
+         int nout, const double *t_out, double *kin, double *pot, long long *blocks,
+         long long *steps, double *dtmin)
+{
+    return ph4h_run_sync(n, m, x, v, eps2, eta, nout, t_out, kin, pot, blocks, steps, dtmin, 0);
 }

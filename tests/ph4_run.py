@@ -5,9 +5,13 @@ JSON; run in its own process because each library holds one global unit.
     python tests/ph4_run.py --integrator ph4|own --lib PATH --ic line7 \\
         --eps2 1e-10 --eta 0.14 --touts 0,0.23,0.24,0.26,0.27
 
-ph4  the reference ph4 integrator (built unchanged by `make -C oracle ph4run`
-     into oracle/_ref/libph4run_<backend>.so, driven by oracle/ph4_harness.cc)
-own  amuse_amd/lib/libhermite4.so bound at run time to --lib
+ph4     the reference ph4 integrator (built unchanged by `make -C oracle ph4run`
+        into oracle/_ref/libph4run_<backend>.so, driven by oracle/ph4_harness.cc)
+own     amuse_amd/lib/libhermite4.so bound at run time to --lib
+device  the device-resident driver of amuse_amd/lib/libg6.so (--lib unused)
+
+--sync: every particle synchronised to the system time after each output
+(ph4's synchronize_all) before the energies are taken (ph4 and device).
 
 Initial conditions: a Plummer sphere (amuse_amd.ic.plummer, --n, --seed), or
 `line7`, the seven bodies on a line of src/amuse_ph4/tests/test_ph4.py:973-981
@@ -41,7 +45,8 @@ def line7():
 
 def main():
     ap = argparse.ArgumentParser()
-    ap.add_argument("--integrator", choices=["ph4", "own"], required=True)
+    ap.add_argument("--integrator", choices=["ph4", "own", "device"], required=True)
+    ap.add_argument("--sync", action="store_true")
     ap.add_argument("--lib", required=True)
     ap.add_argument("--ic", choices=["plummer", "line7"], default="plummer")
     ap.add_argument("--n", type=int, default=256)
@@ -65,12 +70,29 @@ def main():
         nt = touts.size
         k, p, dtmin = np.zeros(nt), np.zeros(nt), np.zeros(nt)
         b, s = np.zeros(nt, np.int64), np.zeros(nt, np.int64)
-        rc = lib.ph4h_run(n, m.ctypes.data_as(D), x.ctypes.data_as(D), v.ctypes.data_as(D),
-                          C.c_double(a.eps2), C.c_double(a.eta), nt,
-                          touts.ctypes.data_as(D), k.ctypes.data_as(D), p.ctypes.data_as(D),
-                          b.ctypes.data_as(L), s.ctypes.data_as(L), dtmin.ctypes.data_as(D))
+        rc = lib.ph4h_run_sync(n, m.ctypes.data_as(D), x.ctypes.data_as(D),
+                               v.ctypes.data_as(D), C.c_double(a.eps2), C.c_double(a.eta), nt,
+                               touts.ctypes.data_as(D), k.ctypes.data_as(D),
+                               p.ctypes.data_as(D), b.ctypes.data_as(L), s.ctypes.data_as(L),
+                               dtmin.ctypes.data_as(D), int(a.sync))
         out.update(E=(k + p).tolist(), K=k.tolist(), blocks=b.tolist(), steps=s.tolist(),
                    dtmin=dtmin.tolist(), used_library=rc)
+    elif a.integrator == "device":
+        from amuse_amd.blocks import DeviceHermite4
+        h = DeviceHermite4(precision=0)
+        h.commit(m, x, v, eps2=a.eps2, eta=a.eta)
+        E, K, B, S, DT = [], [], [], [], []
+        for t in touts:
+            h.evolve(t)
+            if a.sync:
+                h.synchronize()
+            kk, uu = h.energy()
+            c = h.counters()
+            E.append(kk + uu); K.append(kk)
+            B.append(c["block_steps"]); S.append(c["particle_steps"])
+            DT.append(float(h.timesteps()[1].min()))
+        h.close()
+        out.update(E=E, K=K, blocks=B, steps=S, dtmin=DT)
     else:
         from amuse_amd.hermite import Hermite4
         h = Hermite4(a.lib)

@@ -91,6 +91,30 @@ def test_matches_host_driver():
     d.close()
 
 
+@pytest.mark.parametrize("n", [7, 513, 1001])
+def test_sizes_not_multiples_of_32(n):
+    """Systems whose state arrays do not fill whole 256-byte rows (the
+    driver pads every array to a 256-byte boundary) follow the host driver
+    like N = 1024 does."""
+    from amuse_amd.hermite import Hermite4
+    m, x, v = ic.plummer(n, 70 + n)
+    print_case(f"device vs host driver N={n} t=1/64")
+    h = Hermite4(precision=0)
+    h.commit(m, x, v, eps2=1e-4, eta=0.01)
+    d = driver()
+    d.commit(m, x, v, eps2=1e-4, eta=0.01)
+    h.evolve(1 / 64)
+    d.evolve(1 / 64)
+    ch, cd = h.counters(), d.counters()
+    assert ch["block_steps"] == cd["block_steps"]
+    assert ch["particle_steps"] == cd["particle_steps"]
+    xh, vh = h.state()
+    xd, vd = d.state()
+    assert np.abs(xd - xh).max() < 1e-12 and np.abs(vd - vh).max() < 1e-11
+    h.close()
+    d.close()
+
+
 def test_budget_split_is_exact():
     m, x, v = ic.plummer(256, 8)
     runs = []

@@ -18,7 +18,7 @@
   differences along one direction each for x0, v0 and m.
 * c2: 4096 clusters (N log-uniform in [16, 2048], Kroupa masses, M = 1 per
   cluster, eps2 = 1e-6), all to t = 1 in mixed precision with autotuned
-  tiles and graph-replayed steps: every cluster reaches t = 1 and the
+  tiles in one persistent launch: every cluster reaches t = 1 and the
   per-cluster |dE/E| (energies from FP64 passes) has median <= 2e-8 and
   90th percentile <= 2e-7 (measured 8.6e-9 and 8.1e-8).
 (c0 to t = 10 is in test_gpu_c0_energy.py.)
@@ -207,14 +207,9 @@ def test_c2_all_clusters_to_t1():
         ens.autotune()
         e0 = ens.energy()
         ens.start(1.0)
-        rounds = 0
-        while True:
-            r = ens.run(steps_per_graph=16, use_graph=True, max_rounds=8192)
-            rounds += r
-            print(f"  c2 {rounds} step rounds", flush=True)
-            if r < 8192:
-                break
-            assert rounds < 400000
+        rounds = ens.run(persistent=True, max_rounds=400000)
+        print(f"  c2 {rounds} step rounds in one persistent launch", flush=True)
+        assert rounds < 400000
         e1 = ens.energy()
         _, _, tc, ns = ens.state()
     finally:

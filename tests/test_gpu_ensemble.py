@@ -5,8 +5,10 @@
 * FP64 integration of clusters in several size classes (so several lanes-
   per-i shapes) follows the oracle: identical step counts, states within
   1e-10 after 1/16 time unit;
-* a hipGraph replay equals eager launches bit for bit, replay after replay;
-* every tile shape gives the same FP64 forces to rounding;
+* a hipGraph replay equals eager launches bit for bit, replay after replay,
+  and so does the whole run in one persistent cooperative launch;
+* every tile shape (lanes per i, j per LDS tile) gives the same FP64 forces
+  to rounding;
 * mixed-precision energy error stays within a small factor of FP64's.
 """
 import numpy as np
@@ -96,6 +98,72 @@ def test_graph_replay_equals_eager():
         assert np.array_equal(p, q)
 
 
+@pytest.mark.parametrize("prec", [1, 0])
+def test_persistent_run_equals_eager(prec):
+    """The whole run in one persistent cooperative launch (phases separated
+    by grid barriers, work list rebuilt on the device as clusters retire)
+    equals eager launches that check for a rebuild after every round, bit
+    for bit: the same lanes per i for every cluster at every round, so the
+    same forces, steps and states; also when the run is split into
+    launches of a few rounds each."""
+    counts, m, x, v = small_ensemble()
+    res = {}
+    for mode in ("eager", "persistent", "persistent-chunks"):
+        print_case(f"ensemble {mode} prec={prec} t=1/8")
+        ens = Ensemble(counts, m, x, v, eps2=EPS2, dt_param=DTP, precision=prec)
+        ens.start(0.125)
+        if mode == "eager":
+            rounds = ens.run(steps_per_graph=1, use_graph=False, max_rounds=MAX_ROUNDS)
+        elif mode == "persistent":
+            rounds = ens.run(persistent=True, max_rounds=MAX_ROUNDS)
+        else:
+            rounds = 0
+            while True:
+                r = ens.run(persistent=True, max_rounds=7)
+                rounds += r
+                if r < 7:
+                    break
+                assert rounds < MAX_ROUNDS
+        res[mode] = (ens.state(), rounds)
+        ens.close()
+    (se, re_), (sp, rp), (sc, rc) = res["eager"], res["persistent"], res["persistent-chunks"]
+    assert re_ == rp == rc, (re_, rp, rc)
+    assert (se[2] == 0.125).all()
+    for p, q, r in zip(se, sp, sc):
+        assert np.array_equal(p, q)
+        assert np.array_equal(p, r)
+
+
+def test_persistent_run_many_clusters():
+    """256 clusters of 16..512 particles in mixed precision to t = 1/16: the
+    persistent launch equals eager launches bit for bit with autotuned lanes
+    per i and j-tile lengths, and every cluster finishes."""
+    counts, m, x, v = make_ensemble(256, 17, nmin=16, nmax=512)
+    res = []
+    tiles = None
+    for mode in ("eager", "persistent"):
+        print_case(f"ensemble 256 clusters {mode}")
+        ens = Ensemble(counts, m, x, v, eps2=1e-6, dt_param=DTP, precision=1)
+        if tiles is None:
+            assert ens.autotune() > 0
+            tiles = (ens.tiles(), ens.jtiles())
+        else:
+            ens.set_tiles(tiles[0])
+            ens.set_jtiles(tiles[1])
+        ens.start(0.0625)
+        if mode == "eager":
+            rounds = ens.run(steps_per_graph=1, use_graph=False, max_rounds=MAX_ROUNDS)
+        else:
+            rounds = ens.run(persistent=True, max_rounds=MAX_ROUNDS)
+        res.append((ens.state(), rounds))
+        ens.close()
+    print("tiles", tiles, "rounds", res[0][1], flush=True)
+    assert res[0][1] == res[1][1]
+    assert (res[1][0][2] == 0.0625).all()
+    for p, q in zip(res[0][0], res[1][0]):
+        assert np.array_equal(p, q)
+
+
 def test_tile_shapes_agree():
     counts, m, x, v = small_ensemble()
     ens = Ensemble(counts, m, x, v, eps2=EPS2, dt_param=DTP, precision=0)
@@ -105,6 +173,13 @@ def test_tile_shapes_agree():
         got = ens.forces(0)
         for g, r in zip(got, ref):
             assert np.abs(g - r).max() <= 1e-13 * np.abs(r).max()
+    for tj in (64, 128):
+        ens.set_tiles([4] * 8)
+        ens.set_jtiles([tj] * 8)
+        got = ens.forces(0)
+        for g, r in zip(got, ref):
+            assert np.abs(g - r).max() <= 1e-13 * np.abs(r).max()
+    ens.set_jtiles([256] * 8)
     assert ens.autotune() > 0
     tuned = ens.forces(0)
     assert np.abs(tuned[0] - ref[0]).max() <= 1e-13 * np.abs(ref[0]).max()

@@ -98,7 +98,8 @@ def test_golden_reference_vectors(g6):
 def plummer_state(n, seed, dt_nonzero=True):
     from amuse_amd import ic
     rng = np.random.Generator(np.random.PCG64(seed))
-    m, x, v = ic.plummer(n, seed)
+    # exact virial scaling (an O(N^2) potential in numpy) only where it is cheap
+    m, x, v = ic.plummer(n, seed, scale=n <= 4096)
     a2 = rng.normal(0, 0.3, (n, 3))
     j6 = rng.normal(0, 0.3, (n, 3))
     k18 = np.zeros((n, 3))
@@ -152,12 +153,16 @@ def test_mixed_nn_ties_on_lattice(g6):
         g6.set_precision(0)
 
 
-def test_mixed_nn_ties_long_items(g6):
-    """Items whose lanes see >= 16384 j keep the mixed nearest neighbour as a
+@pytest.mark.parametrize("prec", [1, 0])
+def test_nn_ties_long_items(g6, prec):
+    """Items whose lanes see >= 16384 j keep the nearest neighbour as a
     running minimum per block and search the slot once at the end (the wave
     takes one lane's block at a time); shorter items compare per pair.  On an
-    integer lattice (every FP32 r^2 exact, six equidistant neighbours per
-    point) in shuffled slot order, both give the reference's first slot."""
+    integer lattice (every r^2 exact in FP32 and FP64, six equidistant
+    neighbours per point) in shuffled slot order, both give the reference's
+    first slot, in both precisions, through the ABI and through full passes
+    over the slots (g6x_calc_self, whose items away from the diagonal skip
+    the id compare)."""
     g = np.stack(np.meshgrid(np.arange(32), np.arange(32), np.arange(40), indexing="ij"),
                  -1).reshape(-1, 3).astype(np.float64)
     nj = g.shape[0]
@@ -173,16 +178,130 @@ def test_mixed_nn_ties_long_items(g6):
     sel = rng.choice(nj, 256, replace=False)
     id_i, xi, vi = ids[sel], x[sel], v[sel]
     _, _, _, onn, _ = oracle.force(id_i, xi, vi, 0.0, ids, m, x, v)
-    g6.set_precision(1)
+    g6.set_precision(prec)
     try:
         # (njs, ipl, nchunk): lanes' j per item 40960, 40960, 20480 (long);
         # 10240, 10240, 2560 (short)
         for njs, ipl, nchunk in [(1, 2, 1), (1, 1, 1), (1, 2, 2), (1, 2, 4), (4, 1, 1),
                                  (16, 1, 1)]:
-            print_case(f"mixed nn lattice nj={nj} njs={njs} ipl={ipl} nchunk={nchunk}")
+            print_case(f"nn lattice prec={prec} nj={nj} njs={njs} ipl={ipl} nchunk={nchunk}")
             g6.set_tiles(njs, ipl, nchunk, 0)
             nn = g6.calc(nj, id_i, xi, vi, 0.0)[3]
             assert np.array_equal(nn, onn), (njs, ipl, nchunk)
+            g6.calc_self(nj, 0, nj, 0.0, nn=True)
+            assert np.array_equal(g6.fetch(nj)[3][sel], onn), ("self", njs, ipl, nchunk)
+    finally:
+        g6.set_tiles(0, 0)
+        g6.set_precision(0)
+
+
+def test_spatial_full_pass(g6):
+    """Mixed full passes of 2^17 slots take the spatial order by default
+    (slots sorted along a Morton curve, far i-block/j-tile pairs with one
+    FP32 subtraction relative to the i-block centre).  For the shapes that
+    have a spatial kernel, with long (running-minimum) and short
+    nearest-neighbour items, the results are within the mixed path's bound
+    of the 80-bit oracle on sampled i, every nearest neighbour is at the
+    minimal r^2 up to FP32 rounding, sum m a = 0 over all slots, and the
+    slot-order pass (g6x_set_spatial(0)) agrees to the same bound; a shape
+    without a spatial kernel runs in slot order, bit for bit."""
+    n, eps2 = 1 << 17, 1e-6
+    st = plummer_state(n, 41, dt_nonzero=False)
+    ids, m, x, v = st["ids"], st["m"], st["x"], st["v"]
+    g6.set_precision(1)
+    upload(g6, x, v, st["a2"], st["j6"], st["k18"], m, st["tj"], ids)
+    g6.set_ti(st["ti"])
+    s = np.linspace(0, n - 1, 192).astype(int)
+    exact = oracle.force_ld(ids[s], x[s], v[s], eps2, ids, m, x, v)
+    _, _, _, onn, or2 = oracle.force(ids[s], x[s], v[s], eps2, ids, m, x, v)
+    ea, ej, ep, sa, sj, sp = exact
+    try:
+        # (njs, ipl, nchunk): lanes' j per item 131072, 32768 (long), 8192
+        # (short); 1 i per lane; (4, 1) has no spatial kernel
+        for shape in [(1, 2, 1), (1, 2, 4), (1, 2, 16), (1, 1, 2), (0, 0, 0), (4, 1, 8)]:
+            print_case(f"spatial full pass n={n} shape={shape}")
+            g6.set_tiles(*shape)
+            res = {}
+            for sp_on in (True, False):
+                g6.set_spatial(sp_on)
+                g6.calc_self(n, 0, n, eps2, nn=True)
+                res[sp_on] = g6.fetch(n)
+            g6.set_spatial(True)
+            if shape[0] == 4:
+                for p_, q_ in zip(res[True], res[False]):
+                    assert np.array_equal(p_, q_)
+            for a, j, p, nn in res.values():
+                for val, e, sm, name in ((a[s], ea, sa, "acc"), (j[s], ej, sj, "jerk"),
+                                         (p[s], ep, sp, "pot")):
+                    d = np.abs(val - e).reshape(len(s), -1)
+                    d = np.sqrt((d * d).sum(1))
+                    assert (d <= 2e-6 * sm).all(), (shape, name, (d / sm).max())
+                dd = x[nn[s] - 1] - x[s]
+                r2 = (dd * dd).sum(1)
+                assert (r2 <= or2 * (1 + 1e-5)).all(), (shape, (r2 / or2).max())
+                assert ((nn >= 1) & (nn <= n) & (nn != ids)).all()
+                ma = np.linalg.norm((m[:, None] * a).sum(0))
+                assert ma <= 1e-5 * (m * np.linalg.norm(a, axis=1)).sum()
+            da = np.linalg.norm(res[True][0] - res[False][0], axis=1)
+            assert (da <= 4e-6 * np.linalg.norm(res[False][0], axis=1) + 1e-300).mean() > 0.999
+    finally:
+        g6.set_tiles(0, 0)
+        g6.set_spatial(True)
+        g6.set_precision(0)
+
+
+@pytest.mark.parametrize("prec", [0, 1])
+def test_self_pass_id_compare_forms(g6, prec):
+    """g6x_calc_self leaves the per-pair id compare out of the work items
+    whose j-range holds none of their i-slots, but only while every slot
+    below nj holds an id and no id is held twice.  Its results equal the ABI
+    path's on the same i-block (which compares ids in every pair) bit for
+    bit: with distinct ids (most items without the compare; long and short
+    items), with one id held by two slots, and with an empty slot (both: the
+    compare everywhere); the latter two also against the oracle, which skips
+    equal ids and empty slots (lib/g6lib/g6lib.c:135-140)."""
+    n = 40000
+    st = plummer_state(n, 31, dt_nonzero=False)
+    ids = st["ids"].copy()
+    g6.set_precision(prec)
+    try:
+        for case in ("distinct", "repeat", "empty"):
+            idj = ids.copy()
+            skip = np.zeros(n, bool)
+            if case == "repeat":
+                idj[30001] = idj[17]
+            if case == "empty":
+                skip[2222] = True
+            g6.reset()
+            upload(g6, st["x"], st["v"], st["a2"], st["j6"], st["k18"], st["m"], st["tj"], idj,
+                   skip=skip)
+            g6.set_ti(st["ti"])
+            for njs, ipl, nchunk in [(1, 2, 2), (1, 2, 8), (4, 1, 4)]:
+                print_case(f"self pass prec={prec} {case} njs={njs} ipl={ipl} nchunk={nchunk}")
+                g6.set_tiles(njs, ipl, nchunk, 0)
+                g6.calc_self(n, 0, n, 1e-4, nn=True)
+                a2_, j2_, p2_, nn2 = g6.fetch(n)
+                lo, hi = 1000, 5096
+                idi = idj[lo:hi].copy()
+                idi[skip[lo:hi]] = -1
+                a, j, p, nn, _ = g6.calc(n, idi, st["x"][lo:hi], st["v"][lo:hi], 1e-4)
+                live = ~skip[lo:hi]
+                np.testing.assert_array_equal(a[live], a2_[lo:hi][live])
+                np.testing.assert_array_equal(j[live], j2_[lo:hi][live])
+                np.testing.assert_array_equal(p[live], p2_[lo:hi][live])
+                assert np.array_equal(nn[live], nn2[lo:hi][live])
+            if case != "distinct" and prec == 0:
+                s = np.array([17, 2222, 30001, 100, 39999])
+                mj = st["m"] * ~skip
+                jid = np.where(skip, -1, idj).astype(np.int32)
+                _, _, _, onn, _ = oracle.force(jid[s], st["x"][s], st["v"][s], 1e-4, jid, mj,
+                                               st["x"], st["v"])
+                keep = ~skip[s]
+                assert np.array_equal(nn2[s][keep], onn[keep])
+                ea, ej, ep, sa, sj, sp = oracle.force_ld(jid[s], st["x"][s], st["v"][s], 1e-4,
+                                                         jid, mj, st["x"], st["v"])
+                d = np.linalg.norm(a2_[s] - ea, axis=1)
+                assert (d[keep] <= 1e-15 * sa[keep]).all()
     finally:
         g6.set_tiles(0, 0)
         g6.set_precision(0)

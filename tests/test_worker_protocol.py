@@ -133,6 +133,33 @@ def test_particles_vectorised(worker):
     assert worker.call("get_number_of_particles")[1]["number_of_particles"] == n + 1
 
 
+def test_setters_before_commit(worker):
+    """set_state / set_mass / set_radius / set_position / set_velocity
+    (src/amuse_ph4/interface.cc:466-580) write the particle; get_* read it
+    back; an unknown id answers -1 and leaves the others alone."""
+    r, out = worker.call("new_particle", np.array([0.5, 0.25]), np.array([1.0, 2.0]),
+                         0.0, 0.0, 0.0, 0.0, 0.0, 0.1, -1)
+    a, b = out["index_of_the_particle"].tolist()
+    assert worker.call("set_state", a, 0.75, 1.5, 2.5, 3.5, 4.5, 5.5, 6.5, 0.2)[0] == 0
+    _, st = worker.call("get_state", a)
+    assert [st[k] for k in ("mass", "x", "y", "z", "vx", "vy", "vz", "radius")] == \
+        [0.75, 1.5, 2.5, 3.5, 4.5, 5.5, 6.5, 0.2]
+    assert worker.call("set_mass", b, 0.125)[0] == 0
+    assert worker.call("set_radius", b, 0.3)[0] == 0
+    assert worker.call("set_position", b, -1.0, -2.0, -3.0)[0] == 0
+    assert worker.call("set_velocity", b, 7.0, 8.0, 9.0)[0] == 0
+    _, st = worker.call("get_state", b)
+    assert [st[k] for k in ("mass", "x", "y", "z", "vx", "vy", "vz", "radius")] == \
+        [0.125, -1.0, -2.0, -3.0, 7.0, 8.0, 9.0, 0.3]
+    r, _ = worker.call("set_mass", np.array([a, 999, b], dtype=np.int32),
+                       np.array([1.0, 2.0, 3.0]))
+    assert r.tolist() == [0, -1, 0]
+    assert worker.call("get_total_mass")[1]["mass"] == 4.0
+    # without a commit there is nothing to recommit or synchronise
+    assert worker.call("recommit_particles")[0] == -1
+    assert worker.call("synchronize_model")[0] == 0
+
+
 def test_errors(worker):
     rep = worker.send(Message(11, 4242, 1))
     assert rep.error and rep.strings == ["unknown function id: 4242\n"]
