@@ -341,10 +341,16 @@ int ensure_partials(int nchunk, int ni)
 // ---------------------------------------------------------------------------
 // j staging and prediction
 // ---------------------------------------------------------------------------
-int flush_stage()
+// may_allocate = false (the allocation-free entry point): pending records
+// that do not fit the staging buffers already there are an error, and
+// without the staging event the stream is synchronised instead of recorded.
+int flush_stage(bool may_allocate = true)
 {
     const size_t n = U.stage.size();
     if (n == 0) return 0;
+    if (!may_allocate && (n > U.h_stage_cap || (n > size_t(kDirectRows) && n > U.d_stage_cap)))
+        return fail(G6X_ERR_NOMEM, "pending set_j_particle records exceed the staging buffers "
+                                   "(the allocation-free call cannot grow them)");
     if (U.ev_stage) HIPCHK(hipEventSynchronize(U.ev_stage), "stage event");
     if (n > U.h_stage_cap) {
         if (U.h_stage) (void)hipHostFree(U.h_stage);
@@ -372,8 +378,12 @@ int flush_stage()
     }
     const bool pv = U.pred_valid && U.pred_prec == U.prec && U.pred_compat == U.compat;
     HIPCHK(g6k::launch_scatter(U.js, rec, int(n), U.prec, pv ? 1 : 0, U.pred_ti, U.compat, U.stream), "scatter");
-    if (!U.ev_stage) HIPCHK(hipEventCreateWithFlags(&U.ev_stage, hipEventDisableTiming), "event");
-    HIPCHK(hipEventRecord(U.ev_stage, U.stream), "stage record");
+    if (!U.ev_stage && !may_allocate) {
+        HIPCHK(hipStreamSynchronize(U.stream), "stage sync");
+    } else {
+        if (!U.ev_stage) HIPCHK(hipEventCreateWithFlags(&U.ev_stage, hipEventDisableTiming), "event");
+        HIPCHK(hipEventRecord(U.ev_stage, U.stream), "stage record");
+    }
     for (const auto &r : U.stage) U.stage_pos[r.addr] = -1;
     U.stage.clear();
     if (!pv) U.pred_valid = false;
@@ -388,13 +398,14 @@ bool pred_current(double t)
     return U.pred_valid && U.pred_ti == t && U.pred_prec == U.prec && U.pred_compat == U.compat;
 }
 
-int ensure_predicted(double t)
+// timed = false: no timing events (they may be created on first use).
+int ensure_predicted(double t, bool timed = true)
 {
     if (pred_current(t)) return 0;
     Unit::Ev ev;
-    timed_begin(ev, 2, 0.0);
+    if (timed) timed_begin(ev, 2, 0.0);
     HIPCHK(g6k::launch_predict(U.js, U.nj_hw, t, U.prec, U.compat, U.k18_nonzero ? 1 : 0, U.stream), "predict");
-    timed_end(ev);
+    if (timed) timed_end(ev);
     U.pred_valid = true;
     U.pred_ti = t;
     U.pred_prec = U.prec;
@@ -577,6 +588,34 @@ int check_open()
     return 0;
 }
 
+// ---------------------------------------------------------------------------
+// Full pass without allocation (g6x_calc_self_noalloc)
+// ---------------------------------------------------------------------------
+// The i-block rows, result rows and j-chunk partials of this entry point live
+// in one static device array per GPU (kept until the process ends).  Host
+// code reaches it only through the address hipGetSymbolAddress gives with
+// that GPU current (scratch_ptr); the array's own name is a host-side
+// stand-in.  Calls share the array: they run in order on the unit's stream,
+// and calls that overlapped on one GPU (another thread, another stream)
+// would race.  Nothing in it needs clearing: every row a call reads is
+// written by that call first.
+constexpr size_t kScratchDoubles = size_t(G6X_NOALLOC_SCRATCH_BYTES) / sizeof(double);
+static_assert(size_t(G6X_NOALLOC_MAX_NI) * 16 <= kScratchDoubles, "rows must fit the scratch");
+static __device__ __align__(256) double scratch[kScratchDoubles];
+constexpr int kScratchDevices = 64;
+double *scratch_ptr[kScratchDevices] = {};
+
+// Once per device, with it current (at g6_open, else at the first call).
+int scratch_setup(int dev)
+{
+    if (dev < 0 || dev >= kScratchDevices) return fail(G6X_ERR_NODEVICE, "device id beyond 63");
+    if (scratch_ptr[dev]) return 0;
+    void *p = nullptr;
+    HIPCHK(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)), "scratch address");
+    scratch_ptr[dev] = static_cast<double *>(p);
+    return 0;
+}
+
 int do_open(int cluster)
 {
     int rc = ensure_device();
@@ -613,6 +652,7 @@ int do_open(int cluster)
     }
     HIPCHK(hipSetDevice(dev), "hipSetDevice");
     U.device = dev;
+    (void)scratch_setup(dev);  // retried by g6x_calc_self_noalloc if it failed
     if (!U.stream) HIPCHK(hipStreamCreateWithFlags(&U.stream, hipStreamNonBlocking), "stream");
 
     const char *p = getenv("G6_PRECISION");
@@ -833,7 +873,15 @@ G6_API int g6calc_lasthalf2_(int *, int *, int *, int index[], double (*)[3], do
 }
 
 G6_API int g6_initialize_jp_buffer_(int *, int *) { return 0; }
-G6_API int g6_flush_jp_buffer_(int *) { return 0; }
+// A no-op in both references (g6lib.c:447-450, sapporoG6lib.cpp:54); here it
+// applies the staged set_j_particle records to the device store now rather
+// than at the next calc (same results), e.g. before a stretch of
+// g6x_calc_self_noalloc calls, which cannot grow the staging buffers.
+G6_API int g6_flush_jp_buffer_(int *)
+{
+    if (!U.open) return 0;
+    return flush_stage();
+}
 
 G6_API int g6_reset_(int *)
 {
@@ -1135,6 +1183,67 @@ G6_API int g6x_calc_self(int nj, int i0, int ni, double eps2, int want_nn)
     }
     HIPCHK(g6k::launch_iblock_self(U.js, U.ib, i0, ni, U.prec, U.stream), "iblock self");
     return run_force(ni, nj, eps2, want_nn != 0, distinct ? i0 : -1);
+}
+
+// ---------------------------------------------------------------------------
+// Full pass without allocation (g6x_calc_self_noalloc; scratch_setup and the
+// array are defined before g6_open)
+// ---------------------------------------------------------------------------
+G6_API int g6x_calc_self_noalloc(int nj, int i0, int ni, double eps2, double acc[][3],
+                                 double jerk[][3], double pot[], int nn[])
+{
+    int rc = check_open();
+    if (rc) return rc;
+    if (ni < 0 || i0 < 0 || nj < 0) return fail(G6X_ERR_ARG, "negative size");
+    if (ni > G6X_NOALLOC_MAX_NI)
+        return fail(G6X_ERR_ARG, "ni beyond G6X_NOALLOC_MAX_NI (the static scratch)");
+    if (nj > U.js.cap || (long long)i0 + ni > U.js.cap)
+        return fail(G6X_ERR_ARG, "slots beyond the j-store's capacity (the allocation-free "
+                                 "call cannot grow it)");
+    U.latched = false;
+    U.launched = false;
+    U.last_ni = 0;  // the results go to the caller's arrays, not to g6x_fetch
+    if (ni == 0) return 0;
+    if (!acc || !jerk || !pot) return fail(G6X_ERR_ARG, "null result array");
+    rc = scratch_setup(U.device);
+    if (rc) return rc;
+    double *base = scratch_ptr[U.device];
+    if (nj > U.nj_hw) U.nj_hw = nj;
+    rc = flush_stage(false);
+    if (rc) return rc;
+    rc = ensure_predicted(U.ti, false);
+    if (rc) return rc;
+    // rows at pitch ni: x3 v3 h2 id | acc3 jerk3 pot nn | partials
+    IBlock b;
+    const size_t S = size_t(ni);
+    b.cap = ni;
+    for (int k = 0; k < 3; k++) {
+        b.x[k] = base + k * S;
+        b.v[k] = base + (3 + k) * S;
+    }
+    b.h2 = base + 6 * S;
+    b.id = reinterpret_cast<int *>(base + 7 * S);
+    b.out = base + 8 * S;
+    b.nn = reinterpret_cast<int *>(base + 15 * S);
+    b.part = base + 16 * S;
+    const long long fields = U.prec == g6k::kF64 ? g6k::kPartFieldsD : g6k::kPartFields;
+    const long long room = (long long)(kScratchDoubles - 16 * S) / fields;
+    b.part_entries = room;
+    ForceCfg cfg = choose_cfg(ni, std::max(nj, 1));
+    if (cfg.nchunk > 1 && (long long)cfg.nchunk * ni > room)
+        cfg = with_chunks(cfg, cfg.nchunk, ni, std::max(nj, 1), room);
+    const bool want_nn = nn != nullptr;
+    HIPCHK(g6k::launch_iblock_self(U.js, b, i0, ni, U.prec, U.stream), "iblock self");
+    const bool distinct = U.id_repeats == 0 && U.n_filled == nj && U.nj_hw == nj;
+    HIPCHK(g6k::launch_force(U.js, b, ni, std::max(nj, 1), eps2, U.prec, want_nn, cfg, U.stream,
+                             false, 0.0, U.compat, distinct ? i0 : -1),
+           "force kernel");
+    if (cfg.nchunk > 1)
+        HIPCHK(g6k::launch_reduce(U.js, b, ni, cfg.nchunk, U.prec, want_nn, U.stream),
+               "reduce kernel");
+    HIPCHK(g6k::launch_rows_to_aos(b, ni, &acc[0][0], &jerk[0][0], pot, nn, U.stream),
+           "result rows");
+    return 0;
 }
 
 G6_API int g6x_fetch(int ni, double acc[][3], double jerk[][3], double pot[], int nn[])

@@ -264,6 +264,10 @@ inline bool pred_force_cfg(const ForceCfg &c, int nj)
 }
 hipError_t launch_reduce(const JStore &s, const IBlock &b, int ni,
                          int nchunk, int prec, bool want_nn, hipStream_t st);
+// The result rows of b (pitch b.cap) into caller-owned device arrays
+// acc[ni][3], jerk[ni][3], pot[ni] and nn[ni] (nn may be null).
+hipError_t launch_rows_to_aos(const IBlock &b, int ni, double *acc, double *jerk,
+                              double *pot, int *nn, hipStream_t st);
 // Neighbour query over the latched i-block: ids of j with unsoftened
 // r^2 <= h2[i] (self and empty slots excluded).  counts[i] is the full count
 // (must be zeroed first); at most kNgbCap ids per i are stored, in no

@@ -190,6 +190,24 @@ __global__ void __launch_bounds__(256) k_iblock_self(JStore s, IBlock b, int i0,
     b.id[i] = s.id[j];
 }
 
+// Result rows out[7][S] (acc xyz, jerk xyz, pot) and nn rows -> [ni][3]
+// acc and jerk, pot[ni] and nn[ni] (nn may be null).
+__global__ void __launch_bounds__(256) k_rows_to_aos(const double *__restrict__ out,
+                                                     const int *__restrict__ nnr, int S, int ni,
+                                                     double *acc, double *jerk, double *pot,
+                                                     int *nn)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= ni) return;
+#pragma unroll
+    for (int k = 0; k < 3; k++) {
+        acc[3 * size_t(i) + k] = out[k * size_t(S) + i];
+        jerk[3 * size_t(i) + k] = out[(3 + k) * size_t(S) + i];
+    }
+    pot[i] = out[6 * size_t(S) + i];
+    if (nn) nn[i] = nnr[i];
+}
+
 // ------------------------------------------------------------------------
 // Pair interaction, plain FP64.
 // ------------------------------------------------------------------------
@@ -1541,6 +1559,15 @@ hipError_t launch_iblock_self(const JStore &s, const IBlock &b, int i0, int ni, 
     if (ni <= 0) return hipSuccess;
     hipLaunchKernelGGL(k_iblock_self, dim3(nblocks(ni, 256)), dim3(256), 0, st, s, b, i0, ni,
                        prec);
+    return hipGetLastError();
+}
+
+hipError_t launch_rows_to_aos(const IBlock &b, int ni, double *acc, double *jerk, double *pot,
+                              int *nn, hipStream_t st)
+{
+    if (ni <= 0) return hipSuccess;
+    hipLaunchKernelGGL(k_rows_to_aos, dim3(nblocks(ni, 256)), dim3(256), 0, st, b.out, b.nn,
+                       b.cap, ni, acc, jerk, pot, nn);
     return hipGetLastError();
 }
 

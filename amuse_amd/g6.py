@@ -83,6 +83,7 @@ def load_library(path: str = LIBPATH) -> C.CDLL:
         "g6x_set_j_particles": (C.c_int, [C.c_int, _I, _I, _D, _D, _D, _D, _D, _D, _D, _D]),
         "g6x_calc_self": (C.c_int, [C.c_int, C.c_int, C.c_int, C.c_double, C.c_int]),
         "g6x_fetch": (C.c_int, [C.c_int, _D, _D, _D, _I]),
+        "g6x_calc_self_noalloc": (C.c_int, [C.c_int, C.c_int, C.c_int, C.c_double, v, v, v, v]),
         "g6x_sync": (C.c_int, []),
         "g6x_set_spatial": (C.c_int, [C.c_int]),
         "g6x_timing": (C.c_int, [C.c_int]),
@@ -241,6 +242,15 @@ class G6:
     def calc_self(self, nj: int, i0: int, ni: int, eps2: float, nn: bool = True) -> None:
         self._check(self.lib.g6x_calc_self(int(nj), int(i0), int(ni), float(eps2), int(nn)),
                     "g6x_calc_self")
+
+    def calc_self_noalloc(self, nj: int, i0: int, ni: int, eps2: float, acc: int, jerk: int,
+                          pot: int, nn: int | None) -> None:
+        """g6x_calc_self_noalloc: acc, jerk, pot, nn are device addresses
+        (e.g. torch tensors' data_ptr()) of [ni][3], [ni][3], [ni] float64
+        and [ni] int32 arrays; nn None for no nearest neighbour.  Runs on the
+        unit's stream: sync() before reading the arrays."""
+        self._check(self.lib.g6x_calc_self_noalloc(int(nj), int(i0), int(ni), float(eps2), acc,
+                                                   jerk, pot, nn), "g6x_calc_self_noalloc")
 
     def fetch(self, ni: int, nn: bool = True):
         acc = np.zeros((ni, 3))

@@ -64,6 +64,25 @@ int g6x_calc_self(int nj, int i0, int ni, double eps2, int want_nn);
 int g6x_fetch(int ni, double acc[][3], double jerk[][3], double pot[],
               int nn[]);
 
+/* g6x_calc_self without allocation.  No counterpart in the reference: for
+ * callers that cannot allocate device memory (code under HIP graph capture,
+ * serving loops, harnesses that forbid hipMalloc).  The same full pass over
+ * j-slots [i0, i0+ni), results written to caller-owned DEVICE arrays acc
+ * [ni][3], jerk [ni][3], pot [ni] and nn [ni] (ids of the nearest j; NULL:
+ * no nearest neighbour), asynchronously on the unit's stream (g6x_sync
+ * waits).  Its i-block, result rows and j-chunk partials use a static device
+ * array of G6X_NOALLOC_SCRATCH_BYTES per GPU, held until the process ends:
+ * ni <= G6X_NOALLOC_MAX_NI, and fewer j-chunks where the partials would not
+ * fit.  It calls no hipMalloc/hipFree: nj and i0 + ni must lie within the
+ * j-store's current capacity and pending set_j_particle records within its
+ * staging buffers (G6X_ERR_ARG / G6X_ERR_NOMEM otherwise).  Calls share the
+ * array and run in order on the unit's stream; overlapping calls on one GPU
+ * (other threads or streams) would race.  g6x_fetch does not see its
+ * results. */
+#define G6X_NOALLOC_SCRATCH_BYTES (64u << 20)
+#define G6X_NOALLOC_MAX_NI (1 << 19)
+int g6x_calc_self_noalloc(int nj, int i0, int ni, double eps2, double acc[][3],
+                          double jerk[][3], double pot[], int nn[]);
 /* Waits for all work of the unit. */
 int g6x_sync(void);
 /* Full passes in spatial order (default 1): a g6x_calc_self over every slot
