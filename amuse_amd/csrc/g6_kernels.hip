@@ -39,6 +39,11 @@ namespace g6k {
 #ifndef G6_F64_UNROLL
 #define G6_F64_UNROLL 2
 #endif
+// spatial-order mixed kernel: its far/near branch per tile already gives the
+// scheduler two loop bodies; unrolled once more it ran 3 % slower at N = 2^20
+#ifndef G6_SP_UNROLL
+#define G6_SP_UNROLL 1
+#endif
 
 // ------------------------------------------------------------------------
 // Memory carving
@@ -905,13 +910,13 @@ __device__ __forceinline__ void force_mx_item(const FParams &p, TileF &tile, int
 // Mixed path over spatially ordered slots (full passes, g6x_calc_self): the
 // i-block and the j-tiles are runs of slots in Morton order, with the
 // bounding box of each tile's positions in p.tbox.  A tile whose box lies at
-// least 2 R from the i-block's box (R: the i-box's half-diagonal) is "far":
+// least R from the i-block's box (R: the i-box's half-diagonal) is "far":
 // it is staged with positions relative to the i-box centre c, rounded once
 // to FP32 (x_j - c in double, then float), and the pair takes
 // dx = (x_j - c) - (x_i - c) in one FP32 subtraction instead of the three of
 // the double-single difference.  Its error is at most 2^-24 (|x_j - c| +
-// |x_i - c|) <= 2^-24 (|dx| + 2 R) <= 2^-23 |dx| per component (|dx| >= 2 R
-// there), the same order as the double-single path's final rounding.  Other
+// |x_i - c|) <= 2^-24 (|dx| + 2 R) <= 3 * 2^-24 |dx| per component (|dx| >= R
+// there), the order of the double-single path's final rounding.  Other
 // tiles take the double-single difference with the id compare.  The slots
 // are distinct (the caller's precondition), so a far tile never holds an i
 // of the block.
@@ -1023,7 +1028,7 @@ __device__ __forceinline__ void force_mx_item_sp(const FParams &p, TileF &tile, 
             const float g = fmaxf(0.0f, fmaxf(p.tbox[6 * tt + k] - hi[k], lo[k] - p.tbox[6 * tt + 3 + k]));
             d2 = __builtin_fmaf(g, g, d2);
         }
-        const bool far = d2 > 0.0f && d2 >= 4.0f * r2box;
+        const bool far = d2 > 0.0f && d2 >= r2box;
         __syncthreads();
         if (t < jn) {
             const int j = tb + t;
@@ -1041,7 +1046,7 @@ __device__ __forceinline__ void force_mx_item_sp(const FParams &p, TileF &tile, 
         for (int jb = js; jb < jn; jb += NJS * kFlush) {
             const int je = min(jn, jb + NJS * kFlush);
             if (far) {
-#pragma unroll G6_MX_UNROLL
+#pragma unroll G6_SP_UNROLL
                 for (int jj = jb; jj < je; jj += NJS) {
                     const float4 a = tile.a[jj], b = tile.b[jj];
 #pragma unroll
@@ -1058,7 +1063,7 @@ __device__ __forceinline__ void force_mx_item_sp(const FParams &p, TileF &tile, 
                     }
                 }
             } else {
-#pragma unroll G6_MX_UNROLL
+#pragma unroll G6_SP_UNROLL
                 for (int jj = jb; jj < je; jj += NJS) {
                     const float4 a = tile.a[jj], b = tile.b[jj], c = tile.c[jj];
 #pragma unroll
