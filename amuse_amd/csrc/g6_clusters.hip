@@ -28,7 +28,7 @@
 // The four kernels of a step contain no memset and no host synchronisation,
 // so a sequence of steps is captured once into a hipGraph and replayed.
 #include "g6_kernels.h"
-#include <hip/hip_cooperative_groups.h>
+#include "g6_gridbar.h"
 #include "../../include/g6x.h"
 
 #include <algorithm>
@@ -77,6 +77,7 @@ struct Dev {
     // covered, boost, built}
     int *wl_c = nullptr, *wl_i0 = nullptr, *wl_njs = nullptr, *corder = nullptr, *ctl = nullptr;
     int wl_cap = 0;
+    unsigned int *bar = nullptr;  // grid barrier of kc_run (g6bar::kWords, zero at each launch)
     int fill_target = 512;  // items a rebuilt list aims at (see build_items)
 };
 
@@ -372,11 +373,14 @@ __global__ void __launch_bounds__(256) kc_take_forces(Dev d)
 
 // Step from the collision time: dt_param x (1/cmax)^(1/4) (interface.cc:
 // 618-637), rounded down to a power of two dividing t and not above
-// dt_limit; never below dt_floor, so the cluster time always advances.
+// dt_limit; never below dt_floor, so the cluster time always advances.  The
+// fourth root is two correctly rounded square roots and a division (not
+// pow(), whose library code costs the persistent kernel ~30 registers), the
+// same operations as the oracle's step rule.
 __device__ __forceinline__ double next_step(const Dev &d, double t, unsigned int cbits)
 {
     const float cm = __uint_as_float(cbits);
-    double cand = cm > 0.0f ? d.dt_param * pow(double(cm), -0.25) : d.dt_limit;
+    double cand = cm > 0.0f ? d.dt_param / sqrt(sqrt(double(cm))) : d.dt_limit;
     if (!(cand > 0.0)) cand = d.dt_floor;
     double s = d.dt_limit;
     for (int it = 0; it < 80 && s > cand && s > d.dt_floor; it++) s *= 0.5;
@@ -464,6 +468,45 @@ __global__ void kc_set_active_count(Dev d)
 __device__ __forceinline__ int load_shared_int(const int *p)
 {
     return __hip_atomic_load(const_cast<int *>(p), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+// Grid barrier (g6_gridbar.h): thread 0 of each workgroup arrives and waits
+// with device-scope atomics; the workgroup's own writes are ordered before it
+// and its later reads after it by __syncthreads.  The host zeroes d.bar before
+// each launch and reports an aborted launch as an error.
+struct DeviceBarOps {
+    static __device__ void release_fence() { __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent"); }
+    static __device__ void acquire_fence() { __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent"); }
+    static __device__ unsigned fetch_add(unsigned *p, unsigned v)
+    {
+        return __hip_atomic_fetch_add(p, v, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT);
+    }
+    static __device__ void store(unsigned *p, unsigned v)
+    {
+        __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    }
+    static __device__ void store_release(unsigned *p, unsigned v)
+    {
+        __hip_atomic_store(p, v, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+    }
+    static __device__ unsigned poll(unsigned *p)
+    {
+        return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    }
+    static __device__ void pause() { __builtin_amdgcn_s_sleep(2); }
+};
+
+// Returns false (in every thread of the workgroup) when the launch was
+// aborted.  Not inlined: the persistent kernel has six barriers, and the
+// spin loops inlined cost it ~10 registers.
+__device__ __noinline__ bool grid_barrier(unsigned int *bar, unsigned int gen)
+{
+    __shared__ int s_ok;
+    __syncthreads();
+    if (threadIdx.x == 0)
+        s_ok = g6bar::arrive_and_wait<DeviceBarOps>(bar, int(blockIdx.x), int(gridDim.x), gen);
+    __syncthreads();
+    return s_ok != 0;
 }
 
 __device__ __forceinline__ int lanes_of(const Dev &d, int n, int boost)
@@ -568,23 +611,22 @@ __global__ void __launch_bounds__(kWG) kc_run(Dev d, long long max_rounds, long 
     __shared__ TileU tile;
     __shared__ int s_item;
     __shared__ int red[kWG / 64];
-    namespace cg = cooperative_groups;
-    cg::grid_group grid = cg::this_grid();
     const int t = threadIdx.x;
+    unsigned gen = 0;
     long long r = 0;
     // The list state as every workgroup keeps it in registers: read by all
     // before the barrier, written (by workgroup 0, in a rebuild) only after
     // it, and read again after each rebuild's own barrier.
     int built = load_shared_int(&d.ctl[3]);
     int covered = load_shared_int(&d.ctl[1]);
-    grid.sync();
-    for (;;) {
+    bool ok = grid_barrier(d.bar, gen++);
+    while (ok) {
         // work list: none yet, or half the covered clusters retired (nactive
         // changes only in the update phase, three barriers below)
         const int nact = load_shared_int(d.nactive);
         if (!built || nact * 2 <= covered) {
             if (blockIdx.x == 0) build_work_list(d, built, red);
-            grid.sync();
+            if (!grid_barrier(d.bar, gen++)) break;
             built = 1;
             covered = load_shared_int(&d.ctl[1]);
         }
@@ -596,7 +638,7 @@ __global__ void __launch_bounds__(kWG) kc_run(Dev d, long long max_rounds, long 
             const int p = item_particle(d, it, t, c);
             if (p >= 0 && d.active[c]) predict_particle(d, p, d.dt[c]);
         }
-        grid.sync();
+        if (!grid_barrier(d.bar, gen++)) break;
         // force: items off the queue
         for (;;) {
             if (t == 0) s_item = atomicAdd(d.queue, 1);
@@ -608,22 +650,28 @@ __global__ void __launch_bounds__(kWG) kc_run(Dev d, long long max_rounds, long 
             if (!d.active[c]) continue;
             force_item<MIXED>(d, tile, c, d.wl_i0[it], d.wl_njs[it]);
         }
-        grid.sync();
+        if (!grid_barrier(d.bar, gen++)) break;
         // correct
         for (int it = blockIdx.x; it < nwork; it += gridDim.x) {
             int c;
             const int p = item_particle(d, it, t, c);
             if (p >= 0 && d.active[c]) correct_particle(d, p, d.dt[c]);
         }
-        grid.sync();
+        if (!grid_barrier(d.bar, gen++)) break;
         // update: time, next step, retire
         for (int c = blockIdx.x * kWG + t; c < d.nc; c += gridDim.x * kWG)
             update_cluster(d, c, 0);
-        grid.sync();
+        if (!grid_barrier(d.bar, gen++)) break;
         r++;
         if (load_shared_int(d.nactive) <= 0 || r >= max_rounds) break;
     }
     if (blockIdx.x == 0 && t == 0) *rounds_out = r;
+}
+
+__global__ void kc_zero_words(unsigned int *w, int n)
+{
+    for (int k = blockIdx.x * blockDim.x + threadIdx.x; k < n; k += gridDim.x * blockDim.x)
+        w[k] = 0u;
 }
 
 // ---------------------------------------------------------------------------
@@ -862,13 +910,17 @@ G6C_API void *g6c_create(int nc, const int counts[], const double m[], const dou
     std::vector<int> order(nc);
     std::iota(order.begin(), order.end(), 0);
     std::stable_sort(order.begin(), order.end(), [&](int a, int b) { return counts[a] > counts[b]; });
+    // work list, cluster order, then the grid barrier's words on a 256-byte
+    // boundary
+    const size_t bar_at = (3 * size_t(d.wl_cap) + Cn + 63) / 64 * 64;
     if (ok) ok = hipMalloc(reinterpret_cast<void **>(&E->wl_base),
-                           sizeof(int) * (3 * size_t(d.wl_cap) + Cn)) == hipSuccess;
+                           sizeof(int) * (bar_at + g6bar::kWords)) == hipSuccess;
     if (ok) {
         d.wl_c = E->wl_base;
         d.wl_i0 = d.wl_c + d.wl_cap;
         d.wl_njs = d.wl_i0 + d.wl_cap;
         d.corder = d.wl_njs + d.wl_cap;
+        d.bar = reinterpret_cast<unsigned int *>(E->wl_base + bar_at);
         ok &= hipMemcpy(d.corder, order.data(), Cn * 4, hipMemcpyHostToDevice) == hipSuccess;
         const int zero8[8] = {0, 0, 0, 0, 0, 0, 0, 0};
         ok &= hipMemcpy(d.ctl, zero8, sizeof(zero8), hipMemcpyHostToDevice) == hipSuccess;
@@ -937,22 +989,33 @@ static long long run_persistent(Ensemble *E, long long max_rounds)
         E->err = "kc_run: no workgroup fits a compute unit";
         return G6X_ERR_HIP;
     }
-    const int grid = per_cu * cus;
+    const int grid = std::min(per_cu * cus, g6bar::kMaxGroups * g6bar::kGroup);
     long long *d_rounds = reinterpret_cast<long long *>(E->d.ctl + 4);
+    const unsigned int *d_abort = E->d.bar + g6bar::abort_word();
     long long total = 0;
     for (;;) {
         long long budget = 1ll << 20;
         if (max_rounds > 0) budget = std::min(budget, max_rounds - total);
         if (budget <= 0) break;
+        hipLaunchKernelGGL(kc_zero_words, dim3((g6bar::kWords + 255) / 256), dim3(256), 0,
+                           E->stream, E->d.bar, g6bar::kWords);
+        CK(hipGetLastError());
         void *args[] = {&E->d, &budget, &d_rounds};
         CK(hipLaunchCooperativeKernel(kern, dim3(grid), dim3(kWG), args, 0, E->stream));
         long long done = 0;
         int nact = 0;
+        unsigned int aborted = 0;
         CK(hipMemcpyAsync(&done, d_rounds, sizeof(done), hipMemcpyDeviceToHost, E->stream));
         CK(hipMemcpyAsync(&nact, E->d.nactive, sizeof(nact), hipMemcpyDeviceToHost, E->stream));
+        CK(hipMemcpyAsync(&aborted, d_abort, sizeof(aborted), hipMemcpyDeviceToHost, E->stream));
         CK(hipStreamSynchronize(E->stream));
         total += done;
         E->launches++;
+        if (aborted) {
+            E->err = "kc_run: a grid barrier timed out; the run stopped after " +
+                     std::to_string(total) + " rounds";
+            return G6X_ERR_HIP;
+        }
         if (nact <= 0) break;
     }
     return total;

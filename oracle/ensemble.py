@@ -46,7 +46,8 @@ def collision_qmax(m, x, v, eps2):
 
 
 def next_step(t, qmax, dt_param=0.03, dt_limit=0.03125, dt_floor=2.0 ** -40):
-    cand = dt_param * qmax ** -0.25 if qmax > 0 else dt_limit
+    # fourth root as two correctly rounded square roots, as the device does
+    cand = dt_param / math.sqrt(math.sqrt(qmax)) if qmax > 0 else dt_limit
     s = dt_limit
     while s > cand and s > dt_floor:
         s *= 0.5
