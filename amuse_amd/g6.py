@@ -53,9 +53,24 @@ class G6Error(RuntimeError):
     pass
 
 
+def _share_hip_runtime() -> None:
+    """Imports torch, when it is installed, before libg6.so is loaded.
+
+    PyTorch-ROCm ships its own libamdhip64.so under the same soname as
+    ROCm's.  Whichever process loads first provides the HIP runtime to both:
+    torch first, and libg6.so binds to torch's copy and both see the GPU;
+    libg6.so first, and torch later finds no GPU through ROCm's copy.  A
+    process that uses only libg6.so (no torch installed) is unaffected."""
+    try:
+        import torch  # noqa: F401
+    except ImportError:
+        pass
+
+
 def load_library(path: str = LIBPATH) -> C.CDLL:
     if not os.path.exists(path):
         raise G6Error(f"{path} is not built; run amuse_amd.build (hipcc, gfx950)")
+    _share_hip_runtime()
     lib = C.CDLL(path)
     v = C.c_void_p
     sig = {

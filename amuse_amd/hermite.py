@@ -27,6 +27,8 @@ _L = C.POINTER(C.c_longlong)
 def _lib():
     if not os.path.exists(LIBPATH):
         raise RuntimeError(f"{LIBPATH} is not built; run amuse_amd.build")
+    from .g6 import _share_hip_runtime
+    _share_hip_runtime()  # h4_new dlopens the GRAPE-6 library
     L = C.CDLL(LIBPATH)
     L.h4_new.restype = C.c_void_p
     L.h4_new.argtypes = [C.c_char_p, C.c_int, C.c_int, C.c_int, C.c_int]
