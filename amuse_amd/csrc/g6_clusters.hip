@@ -131,7 +131,7 @@ __global__ void __launch_bounds__(256) kc_predict(Dev d, int zero_step)
 struct TileF {
     float4 a[kTJ];  // hx hy hz m
     float4 b[kTJ];  // lx ly lz vx
-    float2 c[kTJ];  // vy vz
+    float4 c[kTJ];  // vy vz - -: the stride of a and b, so one LDS address serves all three
 };
 struct TileD {
     double4 a[kTJ];  // x y z m
@@ -157,6 +157,14 @@ union TileU {
     TileF f;
     TileD dd;
 };
+
+// max of two floats known to be >= +0 or NaN, on their bits: one integer
+// instruction instead of fmaxf's canonicalising pair, and a NaN term wins
+// (then the step rule falls back to dt_limit, as the oracle's max does)
+__device__ __forceinline__ float umax_f(float a, float b)
+{
+    return __uint_as_float(max(__float_as_uint(a), __float_as_uint(b)));
+}
 
 // Work item (cluster c, rows i0 .. i0 + kWG/njs - 1 of it, njs lanes per
 // row) of one force pass; every thread of the workgroup takes part (tile
@@ -197,7 +205,7 @@ __device__ __forceinline__ void force_item(const Dev &d, TileU &tile, int c, int
                 if (MIXED) {
                     tile.f.a[t] = make_float4(d.hx[0][q], d.hx[1][q], d.hx[2][q], d.fm[q]);
                     tile.f.b[t] = make_float4(d.lx[0][q], d.lx[1][q], d.lx[2][q], d.fv[0][q]);
-                    tile.f.c[t] = make_float2(d.fv[1][q], d.fv[2][q]);
+                    tile.f.c[t] = make_float4(d.fv[1][q], d.fv[2][q], 0.0f, 0.0f);
                 } else {
                     tile.dd.a[t] = make_double4(d.px[0][q], d.px[1][q], d.px[2][q], d.m[q]);
                     tile.dd.b[t] = make_double4(d.pv[0][q], d.pv[1][q], d.pv[2][q], 0.0);
@@ -209,7 +217,7 @@ __device__ __forceinline__ void force_item(const Dev &d, TileU &tile, int c, int
                 const bool self = (tb + jj) == i;
                 if (MIXED) {
                     const float4 A = tile.f.a[jj], B = tile.f.b[jj];
-                    const float2 Cc = tile.f.c[jj];
+                    const float4 Cc = tile.f.c[jj];
                     const float dx = (A.x - ihx) + (B.x - ilx);
                     const float dy = (A.y - ihy) + (B.y - ily);
                     const float dz = (A.z - ihz) + (B.z - ilz);
@@ -222,9 +230,12 @@ __device__ __forceinline__ void force_item(const Dev &d, TileU &tile, int c, int
                         // interface.cc:496-516: (s/v^2)^2 and s^4/(m_ij^2 r^2), inverted
                         const float dvx = B.w - ivx, dvy = Cc.x - ivy, dvz = Cc.y - ivz;
                         const float v2 = dvx * dvx + dvy * dvy + dvz * dvz;
-                        cq1 = fmaxf(cq1, v2 * y2);
+                        // both terms are >= +0 (products of squares; s >=
+                        // eps2f as every fma adds a square): their maxima
+                        // are taken on the bits, as the cluster's atomicMax
+                        cq1 = umax_f(cq1, v2 * y2);
                         const float u = (imf + A.w) * y2 * y2;
-                        cq2 = fmaxf(cq2, u * u * (s - eps2f));
+                        cq2 = umax_f(cq2, u * u * (s - eps2f));
                     }
                     fs[0] = __builtin_fmaf(mi3, dx, fs[0]);
                     fs[1] = __builtin_fmaf(mi3, dy, fs[1]);
@@ -298,8 +309,8 @@ __device__ __forceinline__ void force_item(const Dev &d, TileU &tile, int c, int
                 }
             }
         }
-        float cq = fmaxf(cq1 * cq1, cq2);
-        for (int o = 1; o < njs; o <<= 1) cq = fmaxf(cq, __shfl_xor(cq, o));
+        float cq = umax_f(cq1 * cq1, cq2);
+        for (int o = 1; o < njs; o <<= 1) cq = umax_f(cq, __shfl_xor(cq, o));
         if (ok && js == 0) atomicMax(&d.cmax[c], __float_as_uint(cq));
         if (ok && js == 0) {
             double r[7];

@@ -488,10 +488,10 @@ def clusters_c2(args) -> int:
     """Configuration c2: 4096 clusters, N log-uniform in [16, 2048], Kroupa
     0.08-20 normalised to M = 1 per cluster, eps2 = 1e-6, all to t = 1 in
     mixed precision with autotuned tiles (lanes per i and j per tile), the
-    whole run in one persistent cooperative launch (--c2-mode persistent),
-    as hipGraph-captured steps (--c2-mode graph) or as the same step kernels
-    launched one by one (--c2-mode eager); the same ensemble in FP64 for the
-    energy comparison."""
+    step kernels launched one by one (--c2-mode eager, the default),
+    as hipGraph-captured steps (--c2-mode graph) or the whole run in one
+    persistent cooperative launch (--c2-mode persistent); the same ensemble
+    in FP64 for the energy comparison."""
     from amuse_amd.ensemble import Ensemble, make_ensemble
     nc = 4096 if args.clusters <= 0 else args.clusters
     counts, m, x, v = make_ensemble(nc, 2048)
@@ -631,9 +631,10 @@ def main() -> int:
     ap.add_argument("--cpu-threads", type=int, default=0)
     ap.add_argument("--cpu-per-thread", type=int, default=256)
     ap.add_argument("--clusters", type=int, default=0, help="c2: number of clusters")
-    ap.add_argument("--c2-mode", choices=["persistent", "graph", "eager"], default="persistent",
-                    help="c2: one persistent launch for the run, graph-replayed steps, or the "
-                         "same step kernels launched one by one")
+    ap.add_argument("--c2-mode", choices=["persistent", "graph", "eager"], default="eager",
+                    help="c2: the step kernels launched one by one (fastest measured: they run "
+                         "back to back), graph-replayed steps, or one persistent launch for the "
+                         "run (grid barriers between phases)")
     ap.add_argument("--dt-param-c2", type=float, default=0.03,
                     help="c2: hermite dt_param (step = dt_param x collision time)")
     ap.add_argument("--max-rounds", type=int, default=0)
