@@ -63,6 +63,9 @@ __device__ __forceinline__ typename Row4<T>::type row4(T a, T b, T c, T d)
     return r;
 }
 
+__device__ __forceinline__ float fma_t(float a, float b, float c) { return __builtin_fmaf(a, b, c); }
+__device__ __forceinline__ double fma_t(double a, double b, double c) { return __builtin_fma(a, b, c); }
+
 template <class T>
 __device__ __forceinline__ void two_sum(T &hi, T &lo, T x)
 {
@@ -179,8 +182,13 @@ __global__ void __launch_bounds__(kWG) k_adjoint(int n, const T *__restrict__ x,
             const T ss = r0 * r0 + r1 * r1 + r2 * r2 + eps2;
             T y = rsq<T>(ss);
             y = (tb + jj == kc) ? T(0) : y;
-            const T y2 = y * y, y3 = y * y2, y5 = y3 * y2, y7 = y5 * y2;
+            const T y2 = y * y, y3 = y * y2, y5 = y3 * y2;
+            // -3 y^5 and 15 y^7, shared by both views
+            const T y5m3 = T(-3) * y5, y7x15 = T(15) * (y5 * y2);
             const T rv = r0 * w0 + r1 * w1 + r2 * w2;
+            // Every sum takes its terms one fused multiply-add at a time
+            // (s = fma(u, v, s)): the sum is the last operand, so no term
+            // needs a separate add.
             // p's gradients, weight m_k applied at the end
             {
                 const T a0 = Cg.x, a1 = Cg.y, a2 = Cg.z;
@@ -188,34 +196,34 @@ __global__ void __launch_bounds__(kWG) k_adjoint(int n, const T *__restrict__ x,
                 const T rga = r0 * a0 + r1 * a1 + r2 * a2;
                 const T vgj = w0 * b0 + w1 * b1 + w2 * b2;
                 const T rgj = r0 * b0 + r1 * b1 + r2 * b2;
-                const T cr = T(-3) * y5 * (rga + vgj) + T(15) * y7 * rv * rgj;
-                const T cj = T(-3) * y5 * rv;
-                const T cv = T(-3) * y5 * rgj;
-                s[0] += y3 * a0 + cj * b0 + cv * w0 + cr * r0;
-                s[1] += y3 * a1 + cj * b1 + cv * w1 + cr * r1;
-                s[2] += y3 * a2 + cj * b2 + cv * w2 + cr * r2;
-                s[3] += y3 * b0 + cv * r0;
-                s[4] += y3 * b1 + cv * r1;
-                s[5] += y3 * b2 + cv * r2;
+                const T cr = fma_t(y5m3, rga + vgj, y7x15 * rv * rgj);
+                const T cj = y5m3 * rv;
+                const T cv = y5m3 * rgj;
+                s[0] = fma_t(cr, r0, fma_t(cv, w0, fma_t(cj, b0, fma_t(y3, a0, s[0]))));
+                s[1] = fma_t(cr, r1, fma_t(cv, w1, fma_t(cj, b1, fma_t(y3, a1, s[1]))));
+                s[2] = fma_t(cr, r2, fma_t(cv, w2, fma_t(cj, b2, fma_t(y3, a2, s[2]))));
+                s[3] = fma_t(cv, r0, fma_t(y3, b0, s[3]));
+                s[4] = fma_t(cv, r1, fma_t(y3, b1, s[4]));
+                s[5] = fma_t(cv, r2, fma_t(y3, b2, s[5]));
                 // dm_k: -(gA_p . r y3 + gJ_p . (w y3 - 3 rv r y5))
-                s[12] -= rga * y3 + vgj * y3 + cj * rgj;
+                s[12] = fma_t(-cj, rgj, fma_t(-(rga + vgj), y3, s[12]));
             }
             // k's own gradients, weight m_p
             {
                 const T mp = A.w;
-                const T my3 = mp * y3, my5 = mp * y5, my7 = mp * y7;
+                const T my3 = mp * y3, my5m3 = mp * y5m3, my7x15 = mp * y7x15;
                 const T rga = r0 * GA[0] + r1 * GA[1] + r2 * GA[2];
                 const T vgj = w0 * GJ[0] + w1 * GJ[1] + w2 * GJ[2];
                 const T rgj = r0 * GJ[0] + r1 * GJ[1] + r2 * GJ[2];
-                const T cr = T(-3) * my5 * (rga + vgj) + T(15) * my7 * rv * rgj;
-                const T cj = T(-3) * my5 * rv;
-                const T cv = T(-3) * my5 * rgj;
-                s[6] += my3 * GA[0] + cj * GJ[0] + cv * w0 + cr * r0;
-                s[7] += my3 * GA[1] + cj * GJ[1] + cv * w1 + cr * r1;
-                s[8] += my3 * GA[2] + cj * GJ[2] + cv * w2 + cr * r2;
-                s[9] += my3 * GJ[0] + cv * r0;
-                s[10] += my3 * GJ[1] + cv * r1;
-                s[11] += my3 * GJ[2] + cv * r2;
+                const T cr = fma_t(my5m3, rga + vgj, my7x15 * rv * rgj);
+                const T cj = my5m3 * rv;
+                const T cv = my5m3 * rgj;
+                s[6] = fma_t(cr, r0, fma_t(cv, w0, fma_t(cj, GJ[0], fma_t(my3, GA[0], s[6]))));
+                s[7] = fma_t(cr, r1, fma_t(cv, w1, fma_t(cj, GJ[1], fma_t(my3, GA[1], s[7]))));
+                s[8] = fma_t(cr, r2, fma_t(cv, w2, fma_t(cj, GJ[2], fma_t(my3, GA[2], s[8]))));
+                s[9] = fma_t(cv, r0, fma_t(my3, GJ[0], s[9]));
+                s[10] = fma_t(cv, r1, fma_t(my3, GJ[1], s[10]));
+                s[11] = fma_t(cv, r2, fma_t(my3, GJ[2], s[11]));
             }
         }
 #pragma unroll
