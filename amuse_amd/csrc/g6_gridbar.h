@@ -37,13 +37,14 @@ G6BAR_HD constexpr int abort_word() { return (kMaxGroups + 2) * kStride; }
 
 // Called by one thread per workgroup, after the workgroup's own writes are
 // ordered before it.  gen = barriers this workgroup passed in this launch.
-// Returns false when the launch was aborted.  Ops:
+// Returns false when the launch was aborted (SpinLimit polls without the
+// barrier opening, here or in another workgroup).  Ops:
 //   release_fence(), acquire_fence()        agent-scope fences
 //   fetch_add(p, v) -> old                   acquire-release read-modify-write
 //   store(p, v), store_release(p, v)
 //   poll(p)                                  load of a word other workgroups write
 //   pause()                                  back-off between polls
-template <class Ops>
+template <class Ops, unsigned SpinLimit = kSpinLimit>
 G6BAR_HD inline bool arrive_and_wait(unsigned *bar, int block, int grid, unsigned gen)
 {
     const int g = block / kGroup;
@@ -64,7 +65,7 @@ G6BAR_HD inline bool arrive_and_wait(unsigned *bar, int block, int grid, unsigne
     }
     bool ok = true;
     for (unsigned spins = 0; Ops::poll(gw) == gen; spins++) {
-        if (Ops::poll(ab) != 0u || spins >= kSpinLimit) {
+        if (Ops::poll(ab) != 0u || spins >= SpinLimit) {
             Ops::store(ab, 1u);
             ok = false;
             break;

@@ -5,7 +5,9 @@
 // values of this phase (the barrier publishes them), and no slot is written
 // again before the next barrier.  Built with -fsanitize=thread the run also
 // proves the barrier orders those plain accesses (no data race reported).
-// Exit status: 0 all grids passed; 1 a check failed.
+// A grid with a workgroup that never arrives checks the bounded wait: the
+// others give up, raise the abort word and return false.
+// Exit status: 0 all checks passed; 1 a check failed.
 #include <algorithm>
 #include <atomic>
 #include <cstdio>
@@ -27,9 +29,30 @@ struct HostOps {
     static void pause() { std::this_thread::yield(); }
 };
 
+// A workgroup that never arrives: every other one gives up (after its own
+// spin limit, or at once on seeing the abort word another raised), and
+// returns false.  The kernel leaves its loop on the first false.
+static bool abort_path()
+{
+    std::vector<unsigned> bar(g6bar::kWords, 0u);
+    const int grid = 40;  // two groups; workgroup 39 never arrives
+    std::atomic<int> aborted{0}, passed{0};
+    std::vector<std::thread> th;
+    for (int b = 0; b < grid - 1; b++)
+        th.emplace_back([&, b] {
+            if (g6bar::arrive_and_wait<HostOps, 1000u>(bar.data(), b, grid, 0u)) passed++;
+            else aborted++;
+        });
+    for (auto &t : th) t.join();
+    const bool ok = passed == 0 && aborted == grid - 1 && bar[g6bar::abort_word()] == 1u &&
+                    bar[g6bar::gen_word()] == 0u;
+    std::printf("abort path: %s\n", ok ? "ok" : "FAILED");
+    return ok;
+}
+
 int main()
 {
-    int failed = 0;
+    int failed = abort_path() ? 0 : 1;
     std::vector<unsigned> bar(g6bar::kWords);
     for (int grid : {1, 2, 31, 32, 33, 64, 100, 161}) {
         std::fill(bar.begin(), bar.end(), 0u);
