@@ -20,8 +20,9 @@
 // Arrays are [n][3] row-major (torch's contiguous (n, 3)); T is float or
 // double.  Each workgroup covers 256/NJS i-particles, NJS lanes share one
 // i and split the j-range (NJS chosen for ~8 workgroups per CU); j-tiles go
-// through LDS as rows of four values.  Sums are compensated per
-// tile.  No allocation: the caller passes every buffer and its stream.
+// through LDS as rows of four values.  Each lane's plain partial sums are
+// folded into compensated (TwoSum) sums every ~64 pairs.  No allocation:
+// the caller passes every buffer and its stream.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -33,6 +34,12 @@ namespace g6a {
 
 constexpr int kWG = 256;
 constexpr int kTJ = 256;
+// Tiles between folds of a lane's plain partial sums into its compensated
+// ones: ~64 pairs per lane, as in the force kernels (NJS lanes share an i,
+// so a lane takes kTJ / NJS pairs of a tile; folding after every tile cost
+// the FP32 adjoint ~10 instructions per pair at NJS = 32).
+template <int NJS>
+constexpr int kFoldTiles = (64 * NJS) / kTJ > 1 ? (64 * NJS) / kTJ : 1;
 
 template <class T>
 __device__ __forceinline__ T rsq(T s);
@@ -90,7 +97,8 @@ __global__ void __launch_bounds__(kWG) k_forces(int n, const T *__restrict__ x,
     const T xi = x[3 * ic], yi = x[3 * ic + 1], zi = x[3 * ic + 2];
     const T vxi = v[3 * ic], vyi = v[3 * ic + 1], vzi = v[3 * ic + 2];
     T h[6] = {0, 0, 0, 0, 0, 0}, l[6] = {0, 0, 0, 0, 0, 0};
-    for (int tb = 0; tb < n; tb += kTJ) {
+    T s6[6] = {0, 0, 0, 0, 0, 0};
+    for (int tb = 0, nt = 0; tb < n; tb += kTJ) {
         const int jn = min(kTJ, n - tb);
         __syncthreads();
         if (t < jn) {
@@ -99,7 +107,6 @@ __global__ void __launch_bounds__(kWG) k_forces(int n, const T *__restrict__ x,
             sb[t] = row4<T>(v[3 * q], v[3 * q + 1], v[3 * q + 2], T(0));
         }
         __syncthreads();
-        T s6[6] = {0, 0, 0, 0, 0, 0};
         for (int jj = js; jj < jn; jj += NJS) {
             const R A = sa[jj], B = sb[jj];
             const T dx = A.x - xi, dy = A.y - yi, dz = A.z - zi;
@@ -113,8 +120,11 @@ __global__ void __launch_bounds__(kWG) k_forces(int n, const T *__restrict__ x,
             s6[0] += mi3 * dx; s6[1] += mi3 * dy; s6[2] += mi3 * dz;
             s6[3] += mi3 * (dvx + c * dx); s6[4] += mi3 * (dvy + c * dy); s6[5] += mi3 * (dvz + c * dz);
         }
+        if (++nt == kFoldTiles<NJS> || tb + kTJ >= n) {
+            nt = 0;
 #pragma unroll
-        for (int k = 0; k < 6; k++) two_sum(h[k], l[k], s6[k]);
+            for (int k = 0; k < 6; k++) { two_sum(h[k], l[k], s6[k]); s6[k] = 0; }
+        }
     }
 #pragma unroll
     for (int o = 1; o < NJS; o <<= 1)
@@ -161,7 +171,10 @@ __global__ void __launch_bounds__(kWG) k_adjoint(int n, const T *__restrict__ x,
     T h[13], l[13];
 #pragma unroll
     for (int q = 0; q < 13; q++) { h[q] = 0; l[q] = 0; }
-    for (int tb = 0; tb < n; tb += kTJ) {
+    T s[13];
+#pragma unroll
+    for (int q = 0; q < 13; q++) s[q] = 0;
+    for (int tb = 0, nt = 0; tb < n; tb += kTJ) {
         const int jn = min(kTJ, n - tb);
         __syncthreads();
         if (t < jn) {
@@ -172,9 +185,6 @@ __global__ void __launch_bounds__(kWG) k_adjoint(int n, const T *__restrict__ x,
             sd[t] = row4<T>(gj[3 * p], gj[3 * p + 1], gj[3 * p + 2], T(0));
         }
         __syncthreads();
-        T s[13];
-#pragma unroll
-        for (int q = 0; q < 13; q++) s[q] = 0;
         for (int jj = js; jj < jn; jj += NJS) {
             const R A = sa[jj], B = sb[jj], Cg = sc[jj], Dg = sd[jj];
             const T r0 = A.x - X[0], r1 = A.y - X[1], r2 = A.z - X[2];
@@ -226,8 +236,11 @@ __global__ void __launch_bounds__(kWG) k_adjoint(int n, const T *__restrict__ x,
                 s[11] = fma_t(cv, r2, fma_t(my3, GJ[2], s[11]));
             }
         }
+        if (++nt == kFoldTiles<NJS> || tb + kTJ >= n) {
+            nt = 0;
 #pragma unroll
-        for (int q = 0; q < 13; q++) two_sum(h[q], l[q], s[q]);
+            for (int q = 0; q < 13; q++) { two_sum(h[q], l[q], s[q]); s[q] = 0; }
+        }
     }
 #pragma unroll
     for (int o = 1; o < NJS; o <<= 1)
