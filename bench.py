@@ -295,7 +295,7 @@ def ph4_c0(args) -> int:
     (tests/golden/ph4_c0_energy.npz).  Mixed forces: at eta = 0.01 with a
     block budget of 4x the FP64 run's (the step rule's third force difference
     drops below FP32 rounding there), and FP64 and mixed at ph4's default
-    eta = 0.14 to t = 10."""
+    eta = 0.14 to t = 10 (host-side and device-resident drivers)."""
     from amuse_amd import ic
     from amuse_amd.hermite import Hermite4
     gold = np.load(os.path.join(ROOT, "tests", "golden", "ph4_c0_energy.npz"))
@@ -343,6 +343,8 @@ def ph4_c0(args) -> int:
     out["mixed"] = one(1, eta0, budget)
     out["fp64_eta0.14"] = one(0, 0.14)
     out["mixed_eta0.14"] = one(1, 0.14)
+    out["fp64_eta0.14_device_driver"] = one(0, 0.14, device=True)
+    out["mixed_eta0.14_device_driver"] = one(1, 0.14, device=True)
     print(json.dumps(out))
     return 0
 
