@@ -197,24 +197,48 @@ __device__ __forceinline__ void force_item(const Dev &d, TileU &tile, int c, int
         // rounding included) and max(m_ij^2 r^2/s^4)
         float cq1 = 0.0f, cq2 = 0.0f;
         const int tj = d.tj_class[size_class_d(n)];
+        const int lg_njs = __ffs(njs) - 1;  // njs is a power of two
         for (int tb = 0; tb < n; tb += tj) {
             const int jn = min(tj, n - tb);
+            // every lane runs kn iterations over j = js + k njs; entries jn ..
+            // kn njs - 1 of the tile (fewer than njs, at most kTJ in all) are
+            // padding: zero mass at 1e17 (1e100 in FP64) on every axis, where
+            // each term of the pair comes out an exact zero for any i within
+            // ~1e16 of the origin (r^-3 times a zero mass; the collision terms
+            // underflow), so the sums and maxima are those of the jn real j
+            const int kn = (jn + njs - 1) >> lg_njs;
             __syncthreads();
-            if (t < jn) {
-                const int q = off + tb + t;
+            if (t < (kn << lg_njs)) {
+                const bool real = t < jn;
+                const int q = off + tb + (real ? t : 0);
                 if (MIXED) {
-                    tile.f.a[t] = make_float4(d.hx[0][q], d.hx[1][q], d.hx[2][q], d.fm[q]);
-                    tile.f.b[t] = make_float4(d.lx[0][q], d.lx[1][q], d.lx[2][q], d.fv[0][q]);
-                    tile.f.c[t] = make_float4(d.fv[1][q], d.fv[2][q], 0.0f, 0.0f);
+                    if (real) {
+                        tile.f.a[t] = make_float4(d.hx[0][q], d.hx[1][q], d.hx[2][q], d.fm[q]);
+                        tile.f.b[t] = make_float4(d.lx[0][q], d.lx[1][q], d.lx[2][q], d.fv[0][q]);
+                        tile.f.c[t] = make_float4(d.fv[1][q], d.fv[2][q], 0.0f, 0.0f);
+                    } else {
+                        tile.f.a[t] = make_float4(1e17f, 1e17f, 1e17f, 0.0f);
+                        tile.f.b[t] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+                        tile.f.c[t] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+                    }
                 } else {
-                    tile.dd.a[t] = make_double4(d.px[0][q], d.px[1][q], d.px[2][q], d.m[q]);
-                    tile.dd.b[t] = make_double4(d.pv[0][q], d.pv[1][q], d.pv[2][q], 0.0);
+                    if (real) {
+                        tile.dd.a[t] = make_double4(d.px[0][q], d.px[1][q], d.px[2][q], d.m[q]);
+                        tile.dd.b[t] = make_double4(d.pv[0][q], d.pv[1][q], d.pv[2][q], 0.0);
+                    } else {
+                        tile.dd.a[t] = make_double4(1e100, 1e100, 1e100, 0.0);
+                        tile.dd.b[t] = make_double4(0.0, 0.0, 0.0, 0.0);
+                    }
                 }
             }
             __syncthreads();
+            // the lane's own i sits at tile entry i - tb, i.e. iteration ks
+            const int rel = i - tb - js;
+            const int ks = (rel >= 0 && rel < jn && !(rel & (njs - 1))) ? rel >> lg_njs : -1;
             int cntp = 0;
-            for (int jj = js; jj < jn; jj += njs) {
-                const bool self = (tb + jj) == i;
+            for (int k = 0; k < kn; k++) {
+                const int jj = js + (k << lg_njs);
+                const bool self = k == ks;
                 if (MIXED) {
                     const float4 A = tile.f.a[jj], B = tile.f.b[jj];
                     const float4 Cc = tile.f.c[jj];

@@ -4,11 +4,12 @@
   the 80-bit oracle, mixed within 2e-6 sum|term|;
 * FP64 integration of clusters in several size classes (so several lanes-
   per-i shapes) follows the oracle: identical step counts, states within
-  1e-10 after 1/16 time unit;
+  1e-10 after 1/16 time unit; also with shapes whose LDS tiles are mostly
+  zero-mass padding (64 lanes per i over 16- or 64-entry tiles);
 * a hipGraph replay equals eager launches bit for bit, replay after replay,
   and so does the whole run in one persistent cooperative launch;
 * every tile shape (lanes per i, j per LDS tile) gives the same FP64 forces
-  to rounding;
+  to rounding, and mixed forces within the mixed bound of the oracle;
 * mixed-precision energy error stays within a small factor of FP64's.
 """
 import numpy as np
@@ -65,21 +66,37 @@ def test_ensemble_forces():
     ens.close()
 
 
-def test_ensemble_fp64_integration_matches_oracle():
+@pytest.mark.parametrize("shape", [None, (64, 16), (64, 64), (8, 64)])
+def test_ensemble_fp64_integration_matches_oracle(shape):
+    """Also with lanes per i / j-tile shapes whose tiles are mostly padding
+    (every lane runs the same number of j; the entries past the tile's last
+    j are zero-mass padding that must change no sum and no step)."""
     counts, m, x, v = small_ensemble()
     counts, m, x, v = counts[:5], m[:counts[:5].sum()], x[:counts[:5].sum()], v[:counts[:5].sum()]
     ens = Ensemble(counts, m, x, v, eps2=EPS2, dt_param=DTP, precision=0)
+    if shape is not None:
+        ens.set_tiles([shape[0]] * 8)
+        ens.set_jtiles([shape[1]] * 8)
     ens.start(0.0625)
     ens.run(steps_per_graph=4, use_graph=True, max_rounds=MAX_ROUNDS)
     xg, vg, tg, ng = ens.state()
     ens.close()
     for c, sl in enumerate(per_cluster(counts)):
-        print_case(f"ensemble cluster {c} n={counts[c]}")
-        xo, vo, to, so = oens.integrate_cluster(m[sl], x[sl], v[sl], EPS2, DTP, 0.0625)
+        print_case(f"ensemble cluster {c} n={counts[c]} shape {shape}")
+        xo, vo, to, so = _oracle_run(c, m[sl], x[sl], v[sl])
         assert tg[c] == to == 0.0625
         assert ng[c] == so, (c, ng[c], so)
         assert np.abs(xg[sl] - xo).max() < 1e-10
         assert np.abs(vg[sl] - vo).max() < 1e-9
+
+
+_ORACLE_RUNS = {}
+
+
+def _oracle_run(c, m, x, v):
+    if c not in _ORACLE_RUNS:
+        _ORACLE_RUNS[c] = oens.integrate_cluster(m, x, v, EPS2, DTP, 0.0625)
+    return _ORACLE_RUNS[c]
 
 
 def test_graph_replay_equals_eager():
@@ -183,6 +200,27 @@ def test_tile_shapes_agree():
     assert ens.autotune() > 0
     tuned = ens.forces(0)
     assert np.abs(tuned[0] - ref[0]).max() <= 1e-13 * np.abs(ref[0]).max()
+    ens.close()
+
+
+def test_tile_shapes_mixed_within_bound():
+    """Mixed forces of every lanes-per-i / j-tile shape within the mixed
+    bound of the 80-bit oracle (tiles whose last entries are padding
+    included: clusters of 12..2100 against 1..64 lanes per i)."""
+    counts, m, x, v = small_ensemble()
+    ens = Ensemble(counts, m, x, v, eps2=EPS2, dt_param=DTP, precision=1)
+    exact = []
+    for sl in per_cluster(counts):
+        ids = np.arange(sl.stop - sl.start, dtype=np.int32)
+        exact.append(oracle.force_ld(ids, x[sl], v[sl], EPS2, ids, m[sl], x[sl], v[sl]))
+    for njs, tj in [(n, 256) for n in (1, 2, 4, 8, 16, 32, 64)] + [(64, 16), (16, 64), (2, 128)]:
+        ens.set_tiles([njs] * 8)
+        ens.set_jtiles([tj] * 8)
+        amx, jmx, pmx = ens.forces(1)
+        for sl, (ea, ej, ep, sa, sj, sp) in zip(per_cluster(counts), exact):
+            for val, ex, s, tol in ((amx[sl], ea, sa, 2e-6), (jmx[sl], ej, sj, 1e-5), (pmx[sl], ep, sp, 2e-6)):
+                d = np.abs(val - ex).reshape(len(s), -1).max(1)
+                assert (d <= tol * s).all(), (njs, tj, sl, (d / s).max())
     ens.close()
 
 
