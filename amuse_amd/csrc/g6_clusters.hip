@@ -48,6 +48,7 @@ constexpr int kTJ = 256;
 constexpr int kClasses = 8;  // size classes: <=16, <=32, ..., <=2048 (and above)
 constexpr int kFlushC = 64;  // pairs per lane between folds, FP32 sums
 constexpr int kFlushD = 32;  // the same for FP64 sums (clusters are not the hot FP64 path)
+constexpr double kMaxCoord = 1e15;  // largest |coordinate| g6c_create accepts
 
 struct Dev {
     int nc = 0, np = 0;
@@ -861,6 +862,17 @@ G6C_API void *g6c_create(int nc, const int counts[], const double m[], const dou
     if (nc <= 0) return nullptr;
     int ndev = 0;
     if (hipGetDeviceCount(&ndev) != hipSuccess || ndev <= 0) return nullptr;
+    // coordinates within kMaxCoord of the origin: the force kernel pads its
+    // j-tiles with zero-mass entries at 1e17, whose terms are exact zeros
+    // only for i far inside that
+    long long ntot = 0;
+    for (int c = 0; c < nc; c++) {
+        if (counts[c] <= 0) return nullptr;
+        ntot += counts[c];
+    }
+    if (ntot > (1ll << 28)) return nullptr;
+    for (long long k = 0; k < 3 * ntot; k++)
+        if (!(std::fabs(x[k]) <= kMaxCoord)) return nullptr;
     Ensemble *E = new Ensemble;
     E->precision = precision;
     E->cnt.assign(counts, counts + nc);

@@ -59,7 +59,7 @@ class Ensemble:
                                      v.ctypes.data_as(_D), float(eps2), float(dt_param),
                                      int(precision))
         if not self.h:
-            raise G6Error("g6c_create failed (no GPU or bad sizes)")
+            raise G6Error("g6c_create failed (no GPU, bad sizes, or a coordinate not finite or beyond 1e15)")
         self.n = n
         self.nc = self.counts.size
 

@@ -48,6 +48,22 @@ def per_cluster(counts):
     return [slice(off[c], off[c + 1]) for c in range(len(counts))]
 
 
+def test_create_refuses_far_or_nonfinite_coordinates():
+    """The force kernel pads short j-tiles with zero-mass entries at 1e17,
+    so creation refuses coordinates beyond 1e15 (and NaN/inf); 1e15 itself
+    and a valid ensemble are accepted."""
+    from amuse_amd.g6 import G6Error
+    counts, m, x, v = small_ensemble()
+    for bad in (2e15, -1e16, np.nan, np.inf):
+        xb = x.copy()
+        xb[40, 1] = bad
+        with pytest.raises(G6Error):
+            Ensemble(counts, m, xb, v, eps2=EPS2, dt_param=DTP, precision=1)
+    xb = x.copy()
+    xb[40, 1] = 1e15
+    Ensemble(counts, m, xb, v, eps2=EPS2, dt_param=DTP, precision=1).close()
+
+
 def test_ensemble_forces():
     counts, m, x, v = small_ensemble()
     ens = Ensemble(counts, m, x, v, eps2=EPS2, dt_param=DTP, precision=0)
