@@ -1328,6 +1328,13 @@ G6_API int g6x_autotune(int nj, int ni_max, double eps2)
     const int save_njs = U.tile_njs, save_ipl = U.tile_ipl, save_nc = U.tile_nchunk,
               save_smem = U.tile_smem;
     const bool save_timing = U.timing;
+    hipEvent_t a = nullptr, b = nullptr;
+    hipError_t ee = hipEventCreate(&a);
+    if (ee == hipSuccess) ee = hipEventCreate(&b);
+    if (ee != hipSuccess) {
+        if (a) (void)hipEventDestroy(a);
+        return fail(G6X_ERR_HIP, "event", ee);
+    }
     U.timing = false;
     U.tune.clear();
     // {njs, ipl, smem}
@@ -1335,9 +1342,15 @@ G6_API int g6x_autotune(int nj, int ni_max, double eps2)
                                      {2, 1, 0}, {4, 2, 0}, {4, 1, 0}, {8, 2, 0}, {8, 1, 0},
                                      {16, 1, 0}, {32, 1, 0}, {64, 1, 0}, {128, 1, 0},
                                      {256, 1, 0}};
-    hipEvent_t a = nullptr, b = nullptr;
-    HIPCHK(hipEventCreate(&a), "event");
-    HIPCHK(hipEventCreate(&b), "event");
+    // a failure below still restores the unit's tile settings and timing
+#define TUNECHK(expr, what)                        \
+    do {                                           \
+        const hipError_t e_ = (expr);              \
+        if (e_ != hipSuccess) {                    \
+            rc = fail(G6X_ERR_HIP, what, e_);      \
+            goto done;                             \
+        }                                          \
+    } while (0)
     int timed = 0;
     std::vector<TuneEntry> table;
     for (int ni = 1; ni <= ni_max; ni *= 2) {
@@ -1357,14 +1370,14 @@ G6_API int g6x_autotune(int nj, int ni_max, double eps2)
                 U.tile_nchunk = ncs[v];
                 rc = g6x_calc_self(nj, 0, ni, eps2, 1);  // warm-up / allocation
                 if (rc) goto done;
-                HIPCHK(hipEventRecord(a, U.stream), "record");
+                TUNECHK(hipEventRecord(a, U.stream), "record");
                 const int reps = 3;
                 for (int r = 0; r < reps; r++) {
                     rc = run_force(ni, nj, eps2, true);
                     if (rc) goto done;
                 }
-                HIPCHK(hipEventRecord(b, U.stream), "record");
-                HIPCHK(hipEventSynchronize(b), "sync");
+                TUNECHK(hipEventRecord(b, U.stream), "record");
+                TUNECHK(hipEventSynchronize(b), "sync");
                 float ms = 0.0f;
                 (void)hipEventElapsedTime(&ms, a, b);
                 ms /= reps;
@@ -1386,6 +1399,7 @@ G6_API int g6x_autotune(int nj, int ni_max, double eps2)
     }
     table.back().ni_hi = kMaxPipes;
 done:
+#undef TUNECHK
     (void)hipEventDestroy(a);
     (void)hipEventDestroy(b);
     U.tile_njs = save_njs;

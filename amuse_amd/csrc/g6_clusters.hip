@@ -742,6 +742,17 @@ int size_class(int n) { return size_class_d(n); }
         }                                                              \
     } while (0)
 
+// Two timing events, destroyed with the holder.
+struct EventPair {
+    hipEvent_t a = nullptr, b = nullptr;
+    bool create() { return hipEventCreate(&a) == hipSuccess && hipEventCreate(&b) == hipSuccess; }
+    ~EventPair()
+    {
+        if (a) (void)hipEventDestroy(a);
+        if (b) (void)hipEventDestroy(b);
+    }
+};
+
 // Work items: every (cluster, i-block) pair, i-block = 256/njs rows, sorted
 // by descending cost (rows x cluster size) so the queue ends with the cheap
 // ones.  `only` (host flags, one per cluster) restricts the list to some
@@ -804,7 +815,9 @@ int build_items(Ensemble *E, const int *only = nullptr, bool fill = false)
         (void)hipFree(E->d.item_c);
         E->d.item_c = nullptr;
     }
-    CK(hipMalloc(reinterpret_cast<void **>(&E->d.item_c), sizeof(int) * 3 * size_t(std::max(n, 1))));
+    int *items = nullptr;
+    CK(hipMalloc(reinterpret_cast<void **>(&items), sizeof(int) * 3 * size_t(std::max(n, 1))));
+    E->d.item_c = items;
     E->d.item_i0 = E->d.item_c + n;
     E->d.item_njs = E->d.item_c + 2 * n;
     CK(hipMemcpy(E->d.item_c, a.data(), sizeof(int) * n, hipMemcpyHostToDevice));
@@ -852,6 +865,18 @@ int enqueue_step(Ensemble *E, int prec)
 
 using namespace g6c;
 
+// Frees whatever of E has been created (also a partly created ensemble).
+static void release(Ensemble *E)
+{
+    if (E->stream) (void)hipStreamSynchronize(E->stream);
+    if (E->graph) (void)hipGraphExecDestroy(E->graph);
+    if (E->d.item_c) (void)hipFree(E->d.item_c);
+    if (E->wl_base) (void)hipFree(E->wl_base);
+    if (E->base) (void)hipFree(E->base);
+    if (E->stream) (void)hipStreamDestroy(E->stream);
+    delete E;
+}
+
 // Creates an ensemble of nc clusters; cluster c holds counts[c] particles,
 // consecutive in m[], x[][3], v[][3].  precision: G6X_MIXED or G6X_FP64.
 G6C_API void *g6c_create(int nc, const int counts[], const double m[], const double xv[][3],
@@ -859,7 +884,7 @@ G6C_API void *g6c_create(int nc, const int counts[], const double m[], const dou
 {
     const double *x = &xv[0][0];
     const double *v = &vv[0][0];
-    if (nc <= 0) return nullptr;
+    if (nc <= 0 || (precision != G6X_FP64 && precision != G6X_MIXED)) return nullptr;
     int ndev = 0;
     if (hipGetDeviceCount(&ndev) != hipSuccess || ndev <= 0) return nullptr;
     // coordinates within kMaxCoord of the origin: the force kernel pads its
@@ -879,24 +904,30 @@ G6C_API void *g6c_create(int nc, const int counts[], const double m[], const dou
     E->off.resize(nc);
     long long np = 0;
     for (int c = 0; c < nc; c++) {
-        if (counts[c] <= 0) { delete E; return nullptr; }
         E->off[c] = int(np);
         np += counts[c];
     }
-    if (np > (1ll << 28)) { delete E; return nullptr; }
     Dev &d = E->d;
     d.nc = nc;
     d.np = int(np);
     d.eps2 = eps2;
     d.dt_param = dt_param > 0.0 ? dt_param : 0.03;
     for (int k = 0; k < kClasses; k++) E->d.njs_class[k] = k <= 1 ? 16 : (k <= 3 ? 4 : 1);
-    if (hipStreamCreateWithFlags(&E->stream, hipStreamNonBlocking) != hipSuccess) { delete E; return nullptr; }
+    if (hipStreamCreateWithFlags(&E->stream, hipStreamNonBlocking) != hipSuccess) {
+        E->stream = nullptr;
+        release(E);
+        return nullptr;
+    }
     // one allocation: per-particle doubles (x v a j px pv na nj = 24, m, npot), floats (10),
     // ints (pc plist), per-cluster doubles (t dt energy) + uint (cmax) + ints (off cnt active nsteps)
     const size_t P = size_t(np), Cn = size_t(nc);
     auto al = [](size_t b) { return (b + 255) & ~size_t(255); };
     size_t bytes = al(P * 8) * 26 + al(P * 4) * 12 + al(Cn * 8) * 4 + al(Cn * 4) * 4 + 256;
-    if (hipMalloc(&E->base, bytes) != hipSuccess) { delete E; return nullptr; }
+    if (hipMalloc(&E->base, bytes) != hipSuccess) {
+        E->base = nullptr;
+        release(E);
+        return nullptr;
+    }
     char *q = static_cast<char *>(E->base);
     auto D = [&](double *&p) { p = reinterpret_cast<double *>(q); q += al(P * 8); };
     auto F = [&](float *&p) { p = reinterpret_cast<float *>(q); q += al(P * 4); };
@@ -960,8 +991,11 @@ G6C_API void *g6c_create(int nc, const int counts[], const double m[], const dou
     // work list, cluster order, then the grid barrier's words on a 256-byte
     // boundary
     const size_t bar_at = (3 * size_t(d.wl_cap) + Cn + 63) / 64 * 64;
-    if (ok) ok = hipMalloc(reinterpret_cast<void **>(&E->wl_base),
-                           sizeof(int) * (bar_at + g6bar::kWords)) == hipSuccess;
+    if (ok) {
+        ok = hipMalloc(reinterpret_cast<void **>(&E->wl_base),
+                       sizeof(int) * (bar_at + g6bar::kWords)) == hipSuccess;
+        if (!ok) E->wl_base = nullptr;
+    }
     if (ok) {
         d.wl_c = E->wl_base;
         d.wl_i0 = d.wl_c + d.wl_cap;
@@ -973,9 +1007,7 @@ G6C_API void *g6c_create(int nc, const int counts[], const double m[], const dou
         ok &= hipMemcpy(d.ctl, zero8, sizeof(zero8), hipMemcpyHostToDevice) == hipSuccess;
     }
     if (!ok || build_items(E)) {
-        if (E->wl_base) (void)hipFree(E->wl_base);
-        (void)hipFree(E->base);
-        delete E;
+        release(E);
         return nullptr;
     }
     return E;
@@ -1094,29 +1126,34 @@ G6C_API long long g6c_run(void *h, int steps_per_graph, int use_graph, long long
 {
     Ensemble *E = static_cast<Ensemble *>(h);
     if (steps_per_graph < 1) steps_per_graph = 1;
+    EventPair ev;
+    if (!ev.create()) {
+        E->err = "g6c_run: hipEventCreate failed";
+        return G6X_ERR_HIP;
+    }
     if (use_graph == 2) {
-        hipEvent_t a, b;
-        CK(hipEventCreate(&a));
-        CK(hipEventCreate(&b));
-        (void)hipEventRecord(a, E->stream);
+        (void)hipEventRecord(ev.a, E->stream);
         const long long r = run_persistent(E, max_rounds);
-        (void)hipEventRecord(b, E->stream);
-        (void)hipEventSynchronize(b);
+        (void)hipEventRecord(ev.b, E->stream);
+        (void)hipEventSynchronize(ev.b);
         float ms = 0;
-        (void)hipEventElapsedTime(&ms, a, b);
+        (void)hipEventElapsedTime(&ms, ev.a, ev.b);
         E->force_ms += ms;
-        (void)hipEventDestroy(a);
-        (void)hipEventDestroy(b);
         return r;
     }
-    int *h_active = nullptr;
-    CK(hipHostMalloc(reinterpret_cast<void **>(&h_active), sizeof(int)));
+    // the active count, read back after each launch (pinned, freed on return)
+    struct Pinned {
+        int *p = nullptr;
+        ~Pinned()
+        {
+            if (p) (void)hipHostFree(p);
+        }
+    } pin;
+    CK(hipHostMalloc(reinterpret_cast<void **>(&pin.p), sizeof(int)));
+    int *h_active = pin.p;
     std::vector<int> flags(E->d.nc);
     long long rounds = 0;
-    hipEvent_t a, b;
-    CK(hipEventCreate(&a));
-    CK(hipEventCreate(&b));
-    (void)hipEventRecord(a, E->stream);
+    (void)hipEventRecord(ev.a, E->stream);
     long long rc = 0;
     for (;;) {
         if (max_rounds > 0 && rounds >= max_rounds) break;
@@ -1149,14 +1186,11 @@ G6C_API long long g6c_run(void *h, int steps_per_graph, int use_graph, long long
             }
         }
     }
-    (void)hipEventRecord(b, E->stream);
-    (void)hipEventSynchronize(b);
+    (void)hipEventRecord(ev.b, E->stream);
+    (void)hipEventSynchronize(ev.b);
     float ms = 0;
-    (void)hipEventElapsedTime(&ms, a, b);
+    (void)hipEventElapsedTime(&ms, ev.a, ev.b);
     E->force_ms += ms;
-    (void)hipEventDestroy(a);
-    (void)hipEventDestroy(b);
-    (void)hipHostFree(h_active);
     return rc < 0 ? rc : rounds;
 }
 
@@ -1260,18 +1294,10 @@ G6C_API int g6c_get_tiles(void *h, int *njs_per_class)
     return kClasses;
 }
 
-// Autotuner: for each size class present, times the force pass restricted
-// to that class's clusters for every (lanes per i, j per tile) pair -- njs
-// in {1, 2, ..., 64} (i-block rows 256/njs), tile 64, 128 or 256 j where
-// shorter than the class's largest cluster -- and keeps the fastest.
-G6C_API int g6c_autotune(void *h)
+// The autotuner's search (times every shape for each size class present,
+// keeps the fastest): the number of shapes timed, or an error code.
+static int tune_classes(Ensemble *E, hipEvent_t a, hipEvent_t b)
 {
-    Ensemble *E = static_cast<Ensemble *>(h);
-    std::vector<int> saved(E->d.nc);
-    CK(hipMemcpy(saved.data(), E->d.active, sizeof(int) * E->d.nc, hipMemcpyDeviceToHost));
-    hipEvent_t a, b;
-    CK(hipEventCreate(&a));
-    CK(hipEventCreate(&b));
     int timed = 0;
     for (int k = 0; k < kClasses; k++) {
         std::vector<int> act(E->d.nc, 0);
@@ -1294,23 +1320,23 @@ G6C_API int g6c_autotune(void *h)
                 E->d.njs_class[k] = njs;
                 if (build_items(E, act.data())) return G6X_ERR_HIP;
                 hipLaunchKernelGGL(kc_predict, dim3(list_blocks(E)), dim3(256), 0, E->stream, E->d, 1);
-                launch_force(E, E->precision);
+                if (launch_force(E, E->precision)) return G6X_ERR_HIP;
                 // the best of two timings of `reps` passes each, reps raised
                 // until a timing covers >= 1 ms (classes of small clusters
                 // take a few µs per pass, where launch jitter would decide)
                 int reps = 3;
                 float ms_best = 1e30f;
                 for (int trial = 0; trial < 2; trial++) {
-                    (void)hipEventRecord(a, E->stream);
+                    CK(hipEventRecord(a, E->stream));
                     for (int r = 0; r < reps; r++) {
                         hipLaunchKernelGGL(kc_predict, dim3(list_blocks(E)), dim3(256), 0,
                                            E->stream, E->d, 1);
-                        launch_force(E, E->precision);
+                        if (launch_force(E, E->precision)) return G6X_ERR_HIP;
                     }
-                    (void)hipEventRecord(b, E->stream);
+                    CK(hipEventRecord(b, E->stream));
                     CK(hipEventSynchronize(b));
                     float ms = 0;
-                    (void)hipEventElapsedTime(&ms, a, b);
+                    CK(hipEventElapsedTime(&ms, a, b));
                     ms_best = std::min(ms_best, ms / float(reps));
                     if (trial == 0 && ms < 1.0f)
                         reps = std::min(64, int(reps * (1.0f / std::max(ms, 1e-3f))) + 1);
@@ -1322,12 +1348,41 @@ G6C_API int g6c_autotune(void *h)
         E->d.njs_class[k] = best_njs;
         E->d.tj_class[k] = best_tj;
     }
-    (void)hipEventDestroy(a);
-    (void)hipEventDestroy(b);
-    if (build_items(E)) return G6X_ERR_HIP;
+    return timed;
+}
+
+// Autotuner: for each size class present, times the force pass restricted
+// to that class's clusters for every (lanes per i, j per tile) pair -- njs
+// in {1, 2, ..., 64} (i-block rows 256/njs), tile 64, 128 or 256 j where
+// shorter than the class's largest cluster -- and keeps the fastest.  The
+// caller's active flags come back as they were, and after an error so do
+// the shape tables.
+G6C_API int g6c_autotune(void *h)
+{
+    Ensemble *E = static_cast<Ensemble *>(h);
+    std::vector<int> saved(E->d.nc);
+    CK(hipMemcpy(saved.data(), E->d.active, sizeof(int) * E->d.nc, hipMemcpyDeviceToHost));
+    int njs0[kClasses], tj0[kClasses];
+    std::copy(E->d.njs_class, E->d.njs_class + kClasses, njs0);
+    std::copy(E->d.tj_class, E->d.tj_class + kClasses, tj0);
+    int rc = G6X_ERR_HIP;
+    {
+        EventPair ev;
+        if (ev.create())
+            rc = tune_classes(E, ev.a, ev.b);
+        else
+            E->err = "g6c_autotune: hipEventCreate failed";
+    }
+    if (rc < 0) {
+        std::copy(njs0, njs0 + kClasses, E->d.njs_class);
+        std::copy(tj0, tj0 + kClasses, E->d.tj_class);
+    }
+    // items of every cluster again, and the caller's active flags
+    const int rb = build_items(E);
     CK(hipMemcpy(E->d.active, saved.data(), sizeof(int) * E->d.nc, hipMemcpyHostToDevice));
     if (invalidate_work_list(E)) return G6X_ERR_HIP;
-    return timed;
+    if (rc < 0) return rc;
+    return rb ? rb : rc;
 }
 
 G6C_API int g6c_set_precision(void *h, int precision)
@@ -1343,12 +1398,5 @@ G6C_API double g6c_elapsed_ms(void *h) { return static_cast<Ensemble *>(h)->forc
 G6C_API void g6c_free(void *h)
 {
     Ensemble *E = static_cast<Ensemble *>(h);
-    if (!E) return;
-    (void)hipStreamSynchronize(E->stream);
-    if (E->graph) (void)hipGraphExecDestroy(E->graph);
-    if (E->d.item_c) (void)hipFree(E->d.item_c);
-    if (E->wl_base) (void)hipFree(E->wl_base);
-    if (E->base) (void)hipFree(E->base);
-    (void)hipStreamDestroy(E->stream);
-    delete E;
+    if (E) release(E);
 }

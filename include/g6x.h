@@ -122,8 +122,9 @@ int g6x_load_tuning(const char *path);
  * dt_param <= 0), rounded down to a power of two dividing the cluster time;
  * all clusters advance together: one persistent force launch per step, the
  * step's kernels captured in a hipGraph.  Returns NULL on failure (no GPU,
- * a count <= 0, more than 2^28 particles, or a coordinate that is not
- * finite or exceeds 1e15 in magnitude). */
+ * a precision other than G6X_FP64 / G6X_MIXED, a count <= 0, more than
+ * 2^28 particles, or a coordinate that is not finite or exceeds 1e15 in
+ * magnitude). */
 void *g6c_create(int nc, const int counts[], const double m[],
                  const double x[][3], const double v[][3], double eps2,
                  double dt_param, int precision);

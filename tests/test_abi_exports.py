@@ -100,3 +100,31 @@ def test_device_driver_rejects_bad_handles_without_gpu():
         h = lib.g6d_create(4, None, m.ctypes.data_as(D), x.ctypes.data_as(D),
                            x.ctypes.data_as(D), 1e-4, 0.01, 0.0625, 0.03125, 8.0, 0, 1, 0)
         assert h is None
+
+
+def test_tuning_file_round_trip(tmp_path):
+    """g6x_load_tuning keeps the well-formed rows of supported shapes and
+    drops the rest; g6x_save_tuning writes back what was kept.  Neither
+    needs a GPU (the table is host state)."""
+    lib = g6mod.load_library()
+    src = tmp_path / "tune.txt"
+    src.write_text("# ni_lo ni_hi njs ipl nchunk smem\n"
+                   "1 64 1 2 0 1\n"         # kept
+                   "65 4096 4 1 3 0\n"      # kept
+                   "1 2 3 1 0 0\n"          # no shape with 3 lanes per i
+                   "1 2 2 1 0 1\n"          # LDS-resident j needs one lane per i
+                   "10 5 1 1 0 0\n"         # empty ni range
+                   "1 2 1 1 -1 0\n"         # negative chunk count
+                   "garbage\n")
+    empty = tmp_path / "empty.txt"
+    empty.write_text("")
+    try:
+        assert lib.g6x_load_tuning(str(src).encode()) == 2
+        out = tmp_path / "out.txt"
+        assert lib.g6x_save_tuning(str(out).encode()) == 2
+        rows = [ln.split() for ln in out.read_text().splitlines() if not ln.startswith("#")]
+        assert rows == [["1", "64", "1", "2", "0", "1"], ["65", "4096", "4", "1", "3", "0"]]
+        assert lib.g6x_load_tuning(str(tmp_path / "missing.txt").encode()) == -3
+        assert lib.g6x_save_tuning(str(tmp_path / "no" / "such" / "dir.txt").encode()) == -3
+    finally:
+        assert lib.g6x_load_tuning(str(empty).encode()) == 0

@@ -344,6 +344,40 @@ def test_tile_shapes_agree(g6):
     g6.set_tiles(0, 0)
 
 
+def test_autotune_and_tuning_file(g6, tmp_path):
+    """g6x_autotune times the shapes for ni = 1, 2, ..., ni_max against the
+    loaded j-set and keeps one row per ni bucket, which the automatic plan
+    then follows: results stay within the plain path's bars (nn exact), and
+    a table saved and loaded again gives the same bits."""
+    s = plummer_state(3000, 23)
+    upload(g6, s["x"], s["v"], s["a2"], s["j6"], s["k18"], s["m"], s["tj"], s["ids"])
+    g6.set_ti(s["ti"])
+    xp, vp = oracle.predict(s["ti"], s["tj"], s["x"], s["v"], s["a2"], s["j6"], s["k18"])
+    empty = str(tmp_path / "empty.txt")
+    open(empty, "w").close()
+    try:
+        print_case("autotune nj=3000 ni_max=64")
+        assert g6.autotune(3000, 64, 1e-4) > 0
+        for ni in (1, 5, 64):
+            sel = np.arange(0, 3000, 3000 // ni)[:ni]
+            id_i, xi, vi = s["ids"][sel], xp[sel], vp[sel]
+            exact = oracle.force_ld(id_i, xi, vi, 1e-4, s["ids"], s["m"], xp, vp)
+            _, _, _, onn, _ = oracle.force(id_i, xi, vi, 1e-4, s["ids"], s["m"], xp, vp)
+            print_case(f"tuned plan ni={ni}")
+            got = g6.calc(3000, id_i, xi, vi, 1e-4)
+            assert np.array_equal(got[3], onn), ni
+            assert_plain_parity(got[0], got[1], got[2], exact)
+            path = str(tmp_path / "tune.txt")
+            assert g6.save_tuning(path) == 7      # buckets ni <= 1, 2, 4, ..., 64
+            assert g6.load_tuning(empty) == 0
+            assert g6.load_tuning(path) == 7
+            again = g6.calc(3000, id_i, xi, vi, 1e-4)
+            for p, q in zip(got[:4], again[:4]):
+                assert np.array_equal(p, q), ni
+    finally:
+        g6.load_tuning(empty)
+
+
 def test_host_io_boundaries(g6):
     """i-blocks and j-stages on both sides of the 4096-row limit below which
     the kernels read inputs and write results in pinned host memory
