@@ -51,7 +51,8 @@ class Ensemble:
         self.counts = np.ascontiguousarray(counts, dtype=np.int32)
         self.m = np.ascontiguousarray(m, dtype=np.float64)
         n = self.m.size
-        assert self.counts.sum() == n
+        if self.counts.ndim != 1 or (self.counts <= 0).any() or self.counts.sum() != n:
+            raise ValueError("counts must be positive and sum to the number of masses")
         x = np.ascontiguousarray(x, dtype=np.float64).reshape(n, 3)
         v = np.ascontiguousarray(v, dtype=np.float64).reshape(n, 3)
         self.h = self.lib.g6c_create(self.counts.size, self.counts.ctypes.data_as(_I),

@@ -128,3 +128,14 @@ def test_tuning_file_round_trip(tmp_path):
         assert lib.g6x_save_tuning(str(tmp_path / "no" / "such" / "dir.txt").encode()) == -3
     finally:
         assert lib.g6x_load_tuning(str(empty).encode()) == 0
+
+
+def test_ensemble_rejects_bad_counts():
+    """Cluster sizes are checked before anything reaches the library."""
+    import numpy as np
+    from amuse_amd.ensemble import Ensemble
+    m = np.ones(4) / 4
+    x = np.zeros((4, 3))
+    for counts in ([2, 1], [2, 0, 2], [[2, 2]], [5, -1]):
+        with pytest.raises(ValueError):
+            Ensemble(counts, m, x, x)
