@@ -187,7 +187,9 @@ class G6:
         ni = idx.size
         xi = _vec(xi, ni)
         vi = _vec(vi, ni)
-        h2 = np.zeros(ni) if h2 is None else np.ascontiguousarray(h2, dtype=np.float64)
+        # one radius per i (reshape raises on a wrong length: the library
+        # reads ni of them)
+        h2 = np.zeros(ni) if h2 is None else np.ascontiguousarray(h2, dtype=np.float64).reshape(ni)
         dummy = np.zeros((max(ni, 1), 3))
         self._last_i = (idx, xi, vi, h2)
         self.lib.g6calc_firsthalf(self.cluster, int(nj), ni, _i(idx), _d(xi), _d(vi), _d(dummy),
