@@ -15,7 +15,9 @@
 //   dL/dv_k = m_k sum_p Gv(r, v; gJ_p)       - sum_p m_p Gv(r, v; gJ_k)
 //   dL/dm_k = - sum_p [gA_p . r y^3 + gJ_p . (v y^3 - 3 (r.v) r y^5)]
 // (the m_k terms are k acting as the source of p's force, the m_p terms k
-// receiving p's force; the sign of r flips between the two views).
+// receiving p's force; the sign of r flips between the two views).  Gr and
+// Gv are linear in (gA, gJ), so both views of a pair are one Gr and one Gv
+// of (m_k gA_p - m_p gA_k, m_k gJ_p - m_p gJ_k).
 //
 // Arrays are [n][3] row-major (torch's contiguous (n, 3)); T is float or
 // double.  Each workgroup covers 256/NJS i-particles, NJS lanes share one
@@ -167,13 +169,19 @@ __global__ void __launch_bounds__(kWG) k_adjoint(int n, const T *__restrict__ x,
         GA[q] = ga[3 * kc + q];
         GJ[q] = gj[3 * kc + q];
     }
-    // hi/lo of: self_r[3] self_v[3] other_r[3] other_v[3] dm
-    T h[13], l[13];
+    const T mk = m[kc];
+    // Gr and Gv are linear in their adjoint arguments, so the two views of
+    // a pair take one evaluation on the combined adjoint
+    //   gA = m_k gA_p - m_p gA_k,   gJ = m_k gJ_p - m_p gJ_k;
+    // of the terms that multiply k's own gA_k, gJ_k only two scalar sums
+    // remain, T1 = sum_p m_p y^3 and T2 = sum_p m_p (-3 y^5 r.v), applied
+    // at the end.  Sums (hi/lo): gx part[3], gv part[3], T1, T2, dm.
+    T h[9], l[9];
 #pragma unroll
-    for (int q = 0; q < 13; q++) { h[q] = 0; l[q] = 0; }
-    T s[13];
+    for (int q = 0; q < 9; q++) { h[q] = 0; l[q] = 0; }
+    T s[9];
 #pragma unroll
-    for (int q = 0; q < 13; q++) s[q] = 0;
+    for (int q = 0; q < 9; q++) s[q] = 0;
     for (int tb = 0, nt = 0; tb < n; tb += kTJ) {
         const int jn = min(kTJ, n - tb);
         __syncthreads();
@@ -193,71 +201,60 @@ __global__ void __launch_bounds__(kWG) k_adjoint(int n, const T *__restrict__ x,
             T y = rsq<T>(ss);
             y = (tb + jj == kc) ? T(0) : y;
             const T y2 = y * y, y3 = y * y2, y5 = y3 * y2;
-            // -3 y^5 and 15 y^7, shared by both views
             const T y5m3 = T(-3) * y5, y7x15 = T(15) * (y5 * y2);
             const T rv = r0 * w0 + r1 * w1 + r2 * w2;
+            const T mp = A.w;
+            // r.gA, v.gJ, r.gJ with p's adjoint inputs and with k's own
+            const T rga_p = r0 * Cg.x + r1 * Cg.y + r2 * Cg.z;
+            const T vgj_p = w0 * Dg.x + w1 * Dg.y + w2 * Dg.z;
+            const T rgj_p = r0 * Dg.x + r1 * Dg.y + r2 * Dg.z;
+            const T rga_k = r0 * GA[0] + r1 * GA[1] + r2 * GA[2];
+            const T vgj_k = w0 * GJ[0] + w1 * GJ[1] + w2 * GJ[2];
+            const T rgj_k = r0 * GJ[0] + r1 * GJ[1] + r2 * GJ[2];
+            const T sp = rga_p + vgj_p;
+            // the same dots of the combined adjoint
+            const T sgc = fma_t(mk, sp, -mp * (rga_k + vgj_k));
+            const T rgj = fma_t(mk, rgj_p, -mp * rgj_k);
+            const T cj = y5m3 * rv;
+            const T cr = fma_t(y5m3, sgc, (y7x15 * rv) * rgj);
+            const T cv = y5m3 * rgj;
+            const T mky3 = mk * y3, mkcj = mk * cj;
             // Every sum takes its terms one fused multiply-add at a time
             // (s = fma(u, v, s)): the sum is the last operand, so no term
             // needs a separate add.
-            // p's gradients, weight m_k applied at the end
-            {
-                const T a0 = Cg.x, a1 = Cg.y, a2 = Cg.z;
-                const T b0 = Dg.x, b1 = Dg.y, b2 = Dg.z;
-                const T rga = r0 * a0 + r1 * a1 + r2 * a2;
-                const T vgj = w0 * b0 + w1 * b1 + w2 * b2;
-                const T rgj = r0 * b0 + r1 * b1 + r2 * b2;
-                const T cr = fma_t(y5m3, rga + vgj, y7x15 * rv * rgj);
-                const T cj = y5m3 * rv;
-                const T cv = y5m3 * rgj;
-                s[0] = fma_t(cr, r0, fma_t(cv, w0, fma_t(cj, b0, fma_t(y3, a0, s[0]))));
-                s[1] = fma_t(cr, r1, fma_t(cv, w1, fma_t(cj, b1, fma_t(y3, a1, s[1]))));
-                s[2] = fma_t(cr, r2, fma_t(cv, w2, fma_t(cj, b2, fma_t(y3, a2, s[2]))));
-                s[3] = fma_t(cv, r0, fma_t(y3, b0, s[3]));
-                s[4] = fma_t(cv, r1, fma_t(y3, b1, s[4]));
-                s[5] = fma_t(cv, r2, fma_t(y3, b2, s[5]));
-                // dm_k: -(gA_p . r y3 + gJ_p . (w y3 - 3 rv r y5))
-                s[12] = fma_t(-cj, rgj, fma_t(-(rga + vgj), y3, s[12]));
-            }
-            // k's own gradients, weight m_p
-            {
-                const T mp = A.w;
-                const T my3 = mp * y3, my5m3 = mp * y5m3, my7x15 = mp * y7x15;
-                const T rga = r0 * GA[0] + r1 * GA[1] + r2 * GA[2];
-                const T vgj = w0 * GJ[0] + w1 * GJ[1] + w2 * GJ[2];
-                const T rgj = r0 * GJ[0] + r1 * GJ[1] + r2 * GJ[2];
-                const T cr = fma_t(my5m3, rga + vgj, my7x15 * rv * rgj);
-                const T cj = my5m3 * rv;
-                const T cv = my5m3 * rgj;
-                s[6] = fma_t(cr, r0, fma_t(cv, w0, fma_t(cj, GJ[0], fma_t(my3, GA[0], s[6]))));
-                s[7] = fma_t(cr, r1, fma_t(cv, w1, fma_t(cj, GJ[1], fma_t(my3, GA[1], s[7]))));
-                s[8] = fma_t(cr, r2, fma_t(cv, w2, fma_t(cj, GJ[2], fma_t(my3, GA[2], s[8]))));
-                s[9] = fma_t(cv, r0, fma_t(my3, GJ[0], s[9]));
-                s[10] = fma_t(cv, r1, fma_t(my3, GJ[1], s[10]));
-                s[11] = fma_t(cv, r2, fma_t(my3, GJ[2], s[11]));
-            }
+            s[0] = fma_t(cr, r0, fma_t(cv, w0, fma_t(mkcj, Dg.x, fma_t(mky3, Cg.x, s[0]))));
+            s[1] = fma_t(cr, r1, fma_t(cv, w1, fma_t(mkcj, Dg.y, fma_t(mky3, Cg.y, s[1]))));
+            s[2] = fma_t(cr, r2, fma_t(cv, w2, fma_t(mkcj, Dg.z, fma_t(mky3, Cg.z, s[2]))));
+            s[3] = fma_t(cv, r0, fma_t(mky3, Dg.x, s[3]));
+            s[4] = fma_t(cv, r1, fma_t(mky3, Dg.y, s[4]));
+            s[5] = fma_t(cv, r2, fma_t(mky3, Dg.z, s[5]));
+            s[6] = fma_t(mp, y3, s[6]);
+            s[7] = fma_t(mp, cj, s[7]);
+            // dm_k: -(gA_p . r y3 + gJ_p . (w y3 - 3 rv r y5))
+            s[8] = fma_t(-cj, rgj_p, fma_t(-sp, y3, s[8]));
         }
         if (++nt == kFoldTiles<NJS> || tb + kTJ >= n) {
             nt = 0;
 #pragma unroll
-            for (int q = 0; q < 13; q++) { two_sum(h[q], l[q], s[q]); s[q] = 0; }
+            for (int q = 0; q < 9; q++) { two_sum(h[q], l[q], s[q]); s[q] = 0; }
         }
     }
 #pragma unroll
     for (int o = 1; o < NJS; o <<= 1)
 #pragma unroll
-        for (int q = 0; q < 13; q++) {
+        for (int q = 0; q < 9; q++) {
             const T oh = __shfl_xor(h[q], o), ol = __shfl_xor(l[q], o);
             two_sum(h[q], l[q], oh);
             l[q] += ol;
         }
     if (js == 0 && k0 < n) {
-        const T mk = m[k0];
+        const T t1 = h[6] + l[6], t2 = h[7] + l[7];
 #pragma unroll
         for (int q = 0; q < 3; q++) {
-            gx[3 * k0 + q] = mk * (h[q] + l[q]) - (h[6 + q] + l[6 + q]);
-            gv[3 * k0 + q] = mk * (h[3 + q] + l[3 + q]) - (h[9 + q] + l[9 + q]);
+            gx[3 * k0 + q] = (h[q] + l[q]) - fma_t(GA[q], t1, GJ[q] * t2);
+            gv[3 * k0 + q] = (h[3 + q] + l[3 + q]) - GJ[q] * t1;
         }
-        gm[k0] = h[12] + l[12];
+        gm[k0] = h[8] + l[8];
     }
 }
 
