@@ -112,7 +112,7 @@ __global__ void __launch_bounds__(kWG) k_forces(int n, const T *__restrict__ x,
         for (int jj = js; jj < jn; jj += NJS) {
             const R A = sa[jj], B = sb[jj];
             const T dx = A.x - xi, dy = A.y - yi, dz = A.z - zi;
-            const T s = dx * dx + dy * dy + dz * dz + eps2;
+            const T s = fma_t(dz, dz, fma_t(dy, dy, fma_t(dx, dx, eps2)));
             T y = rsq<T>(s);
             y = (tb + jj == ic) ? T(0) : y;
             const T y2 = y * y, mi3 = A.w * y * y2;
@@ -197,7 +197,7 @@ __global__ void __launch_bounds__(kWG) k_adjoint(int n, const T *__restrict__ x,
             const R A = sa[jj], B = sb[jj], Cg = sc[jj], Dg = sd[jj];
             const T r0 = A.x - X[0], r1 = A.y - X[1], r2 = A.z - X[2];
             const T w0 = B.x - V[0], w1 = B.y - V[1], w2 = B.z - V[2];
-            const T ss = r0 * r0 + r1 * r1 + r2 * r2 + eps2;
+            const T ss = fma_t(r2, r2, fma_t(r1, r1, fma_t(r0, r0, eps2)));
             T y = rsq<T>(ss);
             y = (tb + jj == kc) ? T(0) : y;
             const T y2 = y * y, y3 = y * y2, y5 = y3 * y2;
