@@ -4,9 +4,11 @@ Each cluster is integrated with the shared-step 4th-order Hermite scheme of
 src/amuse_hermite/interface.cc (predictor :266-281, collision-time estimate
 :496-531, corrector :536-556, step = dt_param x collision time :618-637)
 with the step rounded down to a power of two dividing the cluster time; all
-clusters advance together: the whole run in one persistent cooperative
-launch (run(persistent=True)), or one persistent force launch per step with
-the steps replayed from a hipGraph.
+clusters advance together, a step round at a time: the step kernels
+launched one by one (run(use_graph=False), the fastest here: queued kernels
+run back to back), replayed from a hipGraph (run(use_graph=True)), or the
+whole run in one persistent launch with grid barriers between the phases
+(run(persistent=True)); the three give the same bits.
 """
 from __future__ import annotations
 

@@ -120,8 +120,9 @@ int g6x_load_tuning(const char *path);
  * Each cluster is integrated with the shared-step 4th-order Hermite scheme
  * of the hermite code: step = dt_param x collision time (0.03 when
  * dt_param <= 0), rounded down to a power of two dividing the cluster time;
- * all clusters advance together: one persistent force launch per step, the
- * step's kernels captured in a hipGraph.  Returns NULL on failure (no GPU,
+ * all clusters advance together, one step round (predict, force, correct,
+ * update) for every active cluster at a time; the force launch is a work
+ * queue of (cluster, i-block) items.  Returns NULL on failure (no GPU,
  * a precision other than G6X_FP64 / G6X_MIXED, a count <= 0, more than
  * 2^28 particles, or a coordinate that is not finite or exceeds 1e15 in
  * magnitude). */
@@ -130,8 +131,13 @@ void *g6c_create(int nc, const int counts[], const double m[],
                  double dt_param, int precision);
 /* initial forces and steps; clusters run until t_end */
 int g6c_start(void *ens, double t_end);
-/* step rounds (steps_per_graph per graph launch) until every cluster has
- * reached t_end or max_rounds (0: no limit); returns the rounds run */
+/* step rounds until every cluster has reached t_end or max_rounds (0: no
+ * limit); returns the rounds run.  use_graph 0: the step kernels launched
+ * one by one, steps_per_graph rounds between checks of the active count;
+ * 1: steps_per_graph rounds captured once in a hipGraph and replayed;
+ * 2: the whole run in one persistent launch (the four phases separated by
+ * grid barriers, the work list rebuilt on the device), equal to mode 0 bit
+ * for bit. */
 long long g6c_run(void *ens, int steps_per_graph, int use_graph,
                   long long max_rounds);
 /* per-cluster total energy from an FP64 force pass at the current state */
@@ -142,10 +148,15 @@ int g6c_state(void *ens, double x[][3], double v[][3], double t[],
 /* forces of the current state, unpredicted */
 int g6c_forces(void *ens, int precision, double acc[][3], double jerk[][3],
                double pot[]);
-/* lanes per i-particle for each of 8 size classes (<=16, <=32, ...) */
+/* lanes per i-particle (1..64, a power of two) for each of 8 size classes
+ * (<=16, <=32, ..., <=2048, more) */
 int g6c_set_tiles(void *ens, const int njs_per_class[8]);
 int g6c_get_tiles(void *ens, int njs_per_class[8]);
-/* times njs = 1..64 per size class present and keeps the fastest */
+/* j per LDS tile (16..256, a power of two) for each size class */
+int g6c_set_jtiles(void *ens, const int tj_per_class[8]);
+int g6c_get_jtiles(void *ens, int tj_per_class[8]);
+/* times every (lanes per i, j per tile) pair per size class present and
+ * keeps the fastest; returns the number of shapes timed */
 int g6c_autotune(void *ens);
 int g6c_set_precision(void *ens, int precision);
 double g6c_elapsed_ms(void *ens);
