@@ -21,7 +21,7 @@
 //
 // Arrays are [n][3] row-major (torch's contiguous (n, 3)); T is float or
 // double.  Each workgroup covers 256/NJS i-particles, NJS lanes share one
-// i and split the j-range (NJS chosen for ~8 workgroups per CU); j-tiles go
+// i and split the j-range (NJS from a workgroup target per kernel); j-tiles go
 // through LDS as rows of four values.  Each lane's plain partial sums are
 // folded into compensated (TwoSum) sums every ~64 pairs.  No allocation:
 // the caller passes every buffer and its stream.
@@ -258,12 +258,17 @@ __global__ void __launch_bounds__(kWG) k_adjoint(int n, const T *__restrict__ x,
     }
 }
 
-// Lanes per i: the smallest power of two from 8 that gives ~2048
-// workgroups (8 per CU) for n particles, at most 64 (one wave per i).
-inline int lanes_per_i(int n)
+// Lanes per i: the smallest power of two from 8 that gives `wgs`
+// workgroups of kWG / njs i-rows for n particles, at most 64 (one wave per
+// i).  Timed at n = 16384 over njs = 8..64: the FP32 forward is fastest at
+// 2048 workgroups (8 per CU), the FP64 forward at 1024, the adjoint at 512
+// (FP64 805 against 908 µs at 2048): more lanes per i mean more distinct j
+// read from LDS per wave and a longer cross-lane fold at the end, which
+// costs the heavier kernels more than the extra waves gain.
+inline int lanes_per_i(int n, long long wgs)
 {
     int njs = 8;
-    while (njs < 64 && (long long)n * njs < 2048ll * kWG) njs *= 2;
+    while (njs < 64 && (long long)n * njs < wgs * kWG) njs *= 2;
     return njs;
 }
 
@@ -281,7 +286,7 @@ int launch_forces(int n, const T *x, const T *v, const T *m, T eps2, T *acc, T *
                   hipStream_t st)
 {
     if (n <= 0) return 0;
-    switch (lanes_per_i(n)) {
+    switch (lanes_per_i(n, sizeof(T) == 4 ? 2048 : 1024)) {
     case 8: launch_forces_t<T, 8>(n, x, v, m, eps2, acc, jerk, st); break;
     case 16: launch_forces_t<T, 16>(n, x, v, m, eps2, acc, jerk, st); break;
     case 32: launch_forces_t<T, 32>(n, x, v, m, eps2, acc, jerk, st); break;
@@ -304,7 +309,7 @@ int launch_adjoint(int n, const T *x, const T *v, const T *m, const T *ga, const
                    T *gx, T *gv, T *gm, hipStream_t st)
 {
     if (n <= 0) return 0;
-    switch (lanes_per_i(n)) {
+    switch (lanes_per_i(n, 512)) {
     case 8: launch_adjoint_t<T, 8>(n, x, v, m, ga, gj, eps2, gx, gv, gm, st); break;
     case 16: launch_adjoint_t<T, 16>(n, x, v, m, ga, gj, eps2, gx, gv, gm, st); break;
     case 32: launch_adjoint_t<T, 32>(n, x, v, m, ga, gj, eps2, gx, gv, gm, st); break;
