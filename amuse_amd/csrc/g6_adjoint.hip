@@ -84,6 +84,47 @@ __device__ __forceinline__ void two_sum(T &hi, T &lo, T x)
     hi = s;
 }
 
+// Sums of the NJS lanes of each i-row.  A workgroup holds kWG / NJS rows,
+// row il in lanes t = il + (kWG / NJS) js, js = 0 .. NJS - 1, so a wave
+// holds every row (and reads only 64 NJS / kWG distinct j per step):
+// butterflies over the lanes of a row within the wave, then the waves'
+// totals in wave order through `fold` (LDS no longer read as a j-tile,
+// (kWG / 64) (kWG / NJS) Q 2 values).  Threads t < kWG / NJS end with the
+// totals of row t; the order of the additions is fixed.
+template <class T, int NJS, int Q>
+__device__ __forceinline__ void fold_rows(T (&h)[Q], T (&l)[Q], T *fold)
+{
+    constexpr int RW = kWG / NJS;
+    static_assert(RW <= 64 && 64 % RW == 0, "every wave holds every row");
+#pragma unroll
+    for (int o = 1; o * RW < 64; o <<= 1)
+#pragma unroll
+        for (int q = 0; q < Q; q++) {
+            const T oh = __shfl_xor(h[q], o * RW), ol = __shfl_xor(l[q], o * RW);
+            two_sum(h[q], l[q], oh);
+            l[q] += ol;
+        }
+    const int t = threadIdx.x, w = t >> 6, u = t & 63;
+    __syncthreads();
+    if (u < RW) {
+#pragma unroll
+        for (int q = 0; q < Q; q++) {
+            fold[((w * RW + u) * Q + q) * 2] = h[q];
+            fold[((w * RW + u) * Q + q) * 2 + 1] = l[q];
+        }
+    }
+    __syncthreads();
+    if (t < RW) {
+        for (int ww = 1; ww < kWG / 64; ww++) {
+#pragma unroll
+            for (int q = 0; q < Q; q++) {
+                two_sum(h[q], l[q], fold[((ww * RW + t) * Q + q) * 2]);
+                l[q] += fold[((ww * RW + t) * Q + q) * 2 + 1];
+            }
+        }
+    }
+}
+
 template <class T, int NJS>
 __global__ void __launch_bounds__(kWG) k_forces(int n, const T *__restrict__ x,
                                                 const T *__restrict__ v,
@@ -91,10 +132,18 @@ __global__ void __launch_bounds__(kWG) k_forces(int n, const T *__restrict__ x,
                                                 T *__restrict__ acc, T *__restrict__ jerk)
 {
     using R = typename Row4<T>::type;
-    __shared__ R sa[kTJ], sb[kTJ];  // (x y z m), (vx vy vz -)
+    constexpr int RW = kWG / NJS;  // i-rows of the workgroup
+    // the j-tile (x y z m), (vx vy vz -), then the fold of the rows' lanes
+    __shared__ union {
+        struct {
+            R a[kTJ], b[kTJ];
+        } tile;
+        T fold[(kWG / 64) * RW * 6 * 2];
+    } sh;
+    R *sa = sh.tile.a, *sb = sh.tile.b;
     const int t = threadIdx.x;
-    const int js = t % NJS, il = t / NJS;
-    const int i = blockIdx.x * (kWG / NJS) + il;
+    const int js = t / RW, il = t % RW;
+    const int i = blockIdx.x * RW + il;
     const int ic = i < n ? i : n - 1;
     const T xi = x[3 * ic], yi = x[3 * ic + 1], zi = x[3 * ic + 2];
     const T vxi = v[3 * ic], vyi = v[3 * ic + 1], vzi = v[3 * ic + 2];
@@ -128,15 +177,8 @@ __global__ void __launch_bounds__(kWG) k_forces(int n, const T *__restrict__ x,
             for (int k = 0; k < 6; k++) { two_sum(h[k], l[k], s6[k]); s6[k] = 0; }
         }
     }
-#pragma unroll
-    for (int o = 1; o < NJS; o <<= 1)
-#pragma unroll
-        for (int k = 0; k < 6; k++) {
-            const T oh = __shfl_xor(h[k], o), ol = __shfl_xor(l[k], o);
-            two_sum(h[k], l[k], oh);
-            l[k] += ol;
-        }
-    if (js == 0 && i < n) {
+    fold_rows<T, NJS, 6>(h, l, sh.fold);
+    if (t < RW && i < n) {
 #pragma unroll
         for (int k = 0; k < 3; k++) {
             acc[3 * i + k] = h[k] + l[k];
@@ -155,11 +197,19 @@ __global__ void __launch_bounds__(kWG) k_adjoint(int n, const T *__restrict__ x,
                                                  T *__restrict__ gm)
 {
     using R = typename Row4<T>::type;
-    // (x y z m), (vx vy vz -), (gA), (gJ) of each j
-    __shared__ R sa[kTJ], sb[kTJ], sc[kTJ], sd[kTJ];
+    constexpr int RW = kWG / NJS;  // k-rows of the workgroup
+    // (x y z m), (vx vy vz -), (gA), (gJ) of each j, then the fold of the
+    // rows' lanes
+    __shared__ union {
+        struct {
+            R a[kTJ], b[kTJ], c[kTJ], d[kTJ];
+        } tile;
+        T fold[(kWG / 64) * RW * 9 * 2];
+    } sh;
+    R *sa = sh.tile.a, *sb = sh.tile.b, *sc = sh.tile.c, *sd = sh.tile.d;
     const int t = threadIdx.x;
-    const int js = t % NJS, il = t / NJS;
-    const int k0 = blockIdx.x * (kWG / NJS) + il;
+    const int js = t / RW, il = t % RW;
+    const int k0 = blockIdx.x * RW + il;
     const int kc = k0 < n ? k0 : n - 1;
     T X[3], V[3], GA[3], GJ[3];
 #pragma unroll
@@ -239,15 +289,8 @@ __global__ void __launch_bounds__(kWG) k_adjoint(int n, const T *__restrict__ x,
             for (int q = 0; q < 9; q++) { two_sum(h[q], l[q], s[q]); s[q] = 0; }
         }
     }
-#pragma unroll
-    for (int o = 1; o < NJS; o <<= 1)
-#pragma unroll
-        for (int q = 0; q < 9; q++) {
-            const T oh = __shfl_xor(h[q], o), ol = __shfl_xor(l[q], o);
-            two_sum(h[q], l[q], oh);
-            l[q] += ol;
-        }
-    if (js == 0 && k0 < n) {
+    fold_rows<T, NJS, 9>(h, l, sh.fold);
+    if (t < RW && k0 < n) {
         const T t1 = h[6] + l[6], t2 = h[7] + l[7];
 #pragma unroll
         for (int q = 0; q < 3; q++) {
