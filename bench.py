@@ -17,6 +17,7 @@ records on its own stream around every force launch.
 from __future__ import annotations
 
 import argparse
+import gc
 import json
 import math
 import os
@@ -190,10 +191,15 @@ def blocks_study(args) -> int:
             g6.calc(n, ids[sel], xp[sel], vp[sel], eps2)     # warm-up
             g6.timing(True)
             reps = max(1, min(20, (1 << 22) // ni))
+            # no collector pass inside the timed calls (one of ~45 ms once
+            # landed in the 20 calls of one block size)
+            gc.collect()
+            gc.disable()
             w0 = time.perf_counter()
             for _ in range(reps):
                 out = g6.calc(n, ids[sel], xp[sel], vp[sel], eps2)
             call_us = (time.perf_counter() - w0) / reps * 1e6
+            gc.enable()
             tm = g6.get_timing()
             g6.timing(False)
             kms = tm["force_ms"] / tm["force_launches"]
@@ -219,8 +225,30 @@ def blocks_study(args) -> int:
             f"{row['fp64']['errors']['acc']['rel_max']:.2e} mixed "
             f"{row['mixed']['errors']['acc']['rel_max']:.2e}")
     g6.close()
+    cpu = None
+    if not args.no_cpu:
+        # the g6lib.c restatement on the host cores, same j-set and i-blocks:
+        # each thread's i-share in one call, and (up to 1024 i) one i per call
+        # as the reference's g6_npipes() = 1 (lib/g6lib/g6lib.c:228-231),
+        # every call predicting the whole j-set as g6lib does
+        host = host_info()
+        threads = args.cpu_threads or usable_cores(host)
+        cpu = {"kind": "port", "host": host, "rows": []}
+        for row in rows:
+            ni = row["ni"]
+            th = min(threads, ni)
+            crow = {"k": row["k"], "ni": ni, "threads": th}
+            for label, npipes in (("npipes_share", max(1, ni // th)), ("npipes_1", 1)):
+                if label == "npipes_1" and ni > 1024:
+                    continue
+                sec = oracle.baseline_run(th, npipes, n, ni, ti, eps2, ids, tj, m, x, v, a2, j6)[0]
+                crow[label] = {"seconds": sec, "gpairs_per_s": ni * n / sec / 1e9,
+                               "speedup_of_gpu_abi_call": {
+                                   p: sec / (row[p]["abi_call_us"] * 1e-6) for p in ("fp64", "mixed")}}
+            cpu["rows"].append(crow)
+            log(f"  cpu k={row['k']}: {crow['npipes_share']['gpairs_per_s']:.3f} Gpairs/s on {th} threads")
     print(json.dumps({"workload": "i-blocks 2^k (k=0..16) vs N=65536 Plummer, per-j times, "
-                                  "eps2=1e-4", "rows": rows}))
+                                  "eps2=1e-4", "rows": rows, "cpu_baseline": cpu}))
     return 0
 
 
