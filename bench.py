@@ -465,10 +465,13 @@ def worker_study(args) -> int:
     w = Ph4Worker()
     try:
         w.call("initialize_code")
+        gc.collect()
+        gc.disable()    # no collector pass inside the 2000 round trips
         t0 = time.perf_counter()
         for _ in range(2000):
             w.call("get_eps2")
         res["scalar_call_us"] = (time.perf_counter() - t0) / 2000 * 1e6
+        gc.enable()
         log(f"  worker: scalar call {res['scalar_call_us']:.1f} us")
         n = 65536
         rng = np.random.Generator(np.random.PCG64(9))
