@@ -10,7 +10,9 @@
   and so does the whole run in one persistent cooperative launch;
 * every tile shape (lanes per i, j per LDS tile) gives the same FP64 forces
   to rounding, and mixed forces within the mixed bound of the oracle;
-* mixed-precision energy error stays within a small factor of FP64's.
+* mixed-precision energy error stays within a small factor of FP64's;
+* clusters of 1, 2, 3 and 17 particles: FP64 follows the oracle, a lone
+  particle moves in a straight line.
 """
 import numpy as np
 import pytest
@@ -256,3 +258,46 @@ def test_mixed_energy_error_vs_fp64():
     print("median |dE/E| fp64", np.median(de[0]), "mixed", np.median(de[1]))
     assert np.median(de[0]) < 1e-7
     assert np.median(de[1]) < 10 * np.median(de[0]) + 1e-8
+
+
+def test_tiny_clusters():
+    """Clusters of 1, 2, 3 and 17 particles (single tiles that are mostly
+    padding, a lone particle with no pair at all): FP64 integration follows
+    the oracle (same step counts, states within 1e-10 after 1/16), the lone
+    particle moves in a straight line at dt_limit steps, and the mixed run
+    finishes with finite states."""
+    from amuse_amd import ic
+    rng = np.random.Generator(np.random.PCG64(21))
+    counts = np.array([1, 2, 3, 17], dtype=np.int32)
+    ms, xs, vs = [], [], []
+    for n in counts:
+        mm = rng.uniform(0.5, 1.5, int(n))
+        mm /= mm.sum()
+        if n == 1:
+            x, v = np.array([[0.3, -0.2, 0.1]]), np.array([[0.5, 0.25, -0.125]])
+        else:
+            _, x, v = ic.plummer_raw(int(n), rng)
+            x, v = ic.to_nbody_units(x, v)
+            x, v = ic.center(mm, x, v)
+        ms.append(mm); xs.append(x); vs.append(v)
+    m, x, v = np.concatenate(ms), np.concatenate(xs), np.concatenate(vs)
+    t_end = 0.0625
+    for prec in (0, 1):
+        print_case(f"tiny clusters prec={prec}")
+        ens = Ensemble(counts, m, x, v, eps2=EPS2, dt_param=DTP, precision=prec)
+        ens.start(t_end)
+        ens.run(steps_per_graph=4, use_graph=False, max_rounds=MAX_ROUNDS)
+        xg, vg, tg, ng = ens.state()
+        ens.close()
+        assert (tg == t_end).all()
+        assert np.isfinite(xg).all() and np.isfinite(vg).all()
+        # the lone particle: no force, steps of dt_limit = 1/32
+        assert ng[0] == 2
+        assert np.abs(xg[0] - (x[0] + v[0] * t_end)).max() < 1e-15
+        assert np.array_equal(vg[0], v[0])
+        if prec == 0:
+            for c, sl in enumerate(per_cluster(counts)):
+                xo, vo, to, so = oens.integrate_cluster(m[sl], x[sl], v[sl], EPS2, DTP, t_end)
+                assert ng[c] == so, (c, ng[c], so)
+                assert np.abs(xg[sl] - xo).max() < 1e-10
+                assert np.abs(vg[sl] - vo).max() < 1e-9
