@@ -293,7 +293,9 @@ __device__ __forceinline__ void force_item(const Dev &d, TileU &tile, int c, int
                         const double q1 = (v2 * y2) * (v2 * y2);
                         const double u = (double(imf) + A.w) * y2 * y2;
                         const double q2 = u * u * (s - d.eps2);
-                        cq2 = fmaxf(cq2, float(fmax(q1, q2)));
+                        // both >= +0 or NaN: the maximum on the bits, as
+                        // the mixed path and the cluster's atomicMax
+                        cq2 = umax_f(cq2, float(fmax(q1, q2)));
                     }
                     ds[0] = __builtin_fma(mi3, dx, ds[0]);
                     ds[1] = __builtin_fma(mi3, dy, ds[1]);
