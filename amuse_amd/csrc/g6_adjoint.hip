@@ -125,30 +125,39 @@ __device__ __forceinline__ void fold_rows(T (&h)[Q], T (&l)[Q], T *fold)
     }
 }
 
-template <class T, int NJS>
+// IPL i-particles per lane (rows il + p RW, p < IPL): each j read from LDS
+// serves IPL independent pairs, which the scheduler interleaves.
+template <class T, int NJS, int IPL>
 __global__ void __launch_bounds__(kWG) k_forces(int n, const T *__restrict__ x,
                                                 const T *__restrict__ v,
                                                 const T *__restrict__ m, T eps2,
                                                 T *__restrict__ acc, T *__restrict__ jerk)
 {
     using R = typename Row4<T>::type;
-    constexpr int RW = kWG / NJS;  // i-rows of the workgroup
+    constexpr int RW = kWG / NJS;  // lane rows of the workgroup
+    constexpr int Q = 6 * IPL;     // sums per lane: acc3 jerk3 of each of its i
     // the j-tile (x y z m), (vx vy vz -), then the fold of the rows' lanes
     __shared__ union {
         struct {
             R a[kTJ], b[kTJ];
         } tile;
-        T fold[(kWG / 64) * RW * 6 * 2];
+        T fold[(kWG / 64) * RW * Q * 2];
     } sh;
     R *sa = sh.tile.a, *sb = sh.tile.b;
     const int t = threadIdx.x;
     const int js = t / RW, il = t % RW;
-    const int i = blockIdx.x * RW + il;
-    const int ic = i < n ? i : n - 1;
-    const T xi = x[3 * ic], yi = x[3 * ic + 1], zi = x[3 * ic + 2];
-    const T vxi = v[3 * ic], vyi = v[3 * ic + 1], vzi = v[3 * ic + 2];
-    T h[6] = {0, 0, 0, 0, 0, 0}, l[6] = {0, 0, 0, 0, 0, 0};
-    T s6[6] = {0, 0, 0, 0, 0, 0};
+    int ic[IPL];
+    T xi[IPL], yi[IPL], zi[IPL], vxi[IPL], vyi[IPL], vzi[IPL];
+#pragma unroll
+    for (int p = 0; p < IPL; p++) {
+        const int i = blockIdx.x * RW * IPL + il + p * RW;
+        ic[p] = i < n ? i : n - 1;
+        xi[p] = x[3 * ic[p]]; yi[p] = x[3 * ic[p] + 1]; zi[p] = x[3 * ic[p] + 2];
+        vxi[p] = v[3 * ic[p]]; vyi[p] = v[3 * ic[p] + 1]; vzi[p] = v[3 * ic[p] + 2];
+    }
+    T h[Q], l[Q], s6[Q];
+#pragma unroll
+    for (int k = 0; k < Q; k++) { h[k] = 0; l[k] = 0; s6[k] = 0; }
     for (int tb = 0, nt = 0; tb < n; tb += kTJ) {
         const int jn = min(kTJ, n - tb);
         __syncthreads();
@@ -160,29 +169,38 @@ __global__ void __launch_bounds__(kWG) k_forces(int n, const T *__restrict__ x,
         __syncthreads();
         for (int jj = js; jj < jn; jj += NJS) {
             const R A = sa[jj], B = sb[jj];
-            const T dx = A.x - xi, dy = A.y - yi, dz = A.z - zi;
-            const T s = fma_t(dz, dz, fma_t(dy, dy, fma_t(dx, dx, eps2)));
-            T y = rsq<T>(s);
-            y = (tb + jj == ic) ? T(0) : y;
-            const T y2 = y * y, mi3 = A.w * y * y2;
-            const T dvx = B.x - vxi, dvy = B.y - vyi, dvz = B.z - vzi;
-            const T rv = dx * dvx + dy * dvy + dz * dvz;
-            const T c = T(-3) * rv * y2;
-            s6[0] += mi3 * dx; s6[1] += mi3 * dy; s6[2] += mi3 * dz;
-            s6[3] += mi3 * (dvx + c * dx); s6[4] += mi3 * (dvy + c * dy); s6[5] += mi3 * (dvz + c * dz);
+#pragma unroll
+            for (int p = 0; p < IPL; p++) {
+                const T dx = A.x - xi[p], dy = A.y - yi[p], dz = A.z - zi[p];
+                const T s = fma_t(dz, dz, fma_t(dy, dy, fma_t(dx, dx, eps2)));
+                T y = rsq<T>(s);
+                y = (tb + jj == ic[p]) ? T(0) : y;
+                const T y2 = y * y, mi3 = A.w * y * y2;
+                const T dvx = B.x - vxi[p], dvy = B.y - vyi[p], dvz = B.z - vzi[p];
+                const T rv = dx * dvx + dy * dvy + dz * dvz;
+                const T c = T(-3) * rv * y2;
+                T *sp = s6 + 6 * p;
+                sp[0] += mi3 * dx; sp[1] += mi3 * dy; sp[2] += mi3 * dz;
+                sp[3] += mi3 * (dvx + c * dx); sp[4] += mi3 * (dvy + c * dy); sp[5] += mi3 * (dvz + c * dz);
+            }
         }
         if (++nt == kFoldTiles<NJS> || tb + kTJ >= n) {
             nt = 0;
 #pragma unroll
-            for (int k = 0; k < 6; k++) { two_sum(h[k], l[k], s6[k]); s6[k] = 0; }
+            for (int k = 0; k < Q; k++) { two_sum(h[k], l[k], s6[k]); s6[k] = 0; }
         }
     }
-    fold_rows<T, NJS, 6>(h, l, sh.fold);
-    if (t < RW && i < n) {
+    fold_rows<T, NJS, Q>(h, l, sh.fold);
+    if (t < RW) {
 #pragma unroll
-        for (int k = 0; k < 3; k++) {
-            acc[3 * i + k] = h[k] + l[k];
-            jerk[3 * i + k] = h[3 + k] + l[3 + k];
+        for (int p = 0; p < IPL; p++) {
+            const int i = blockIdx.x * RW * IPL + t + p * RW;
+            if (i >= n) continue;
+#pragma unroll
+            for (int k = 0; k < 3; k++) {
+                acc[3 * i + k] = h[6 * p + k] + l[6 * p + k];
+                jerk[3 * i + k] = h[6 * p + 3 + k] + l[6 * p + 3 + k];
+            }
         }
     }
 }
@@ -302,12 +320,14 @@ __global__ void __launch_bounds__(kWG) k_adjoint(int n, const T *__restrict__ x,
 }
 
 // Lanes per i: the smallest power of two from 8 that gives `wgs`
-// workgroups of kWG / njs i-rows for n particles, at most 64 (one wave per
-// i).  Timed at n = 16384 over njs = 8..64: the FP32 forward is fastest at
-// 2048 workgroups (8 per CU), the FP64 forward at 1024, the adjoint at 512
-// (FP64 805 against 908 µs at 2048): more lanes per i mean more distinct j
-// read from LDS per wave and a longer cross-lane fold at the end, which
-// costs the heavier kernels more than the extra waves gain.
+// workgroups of kWG / njs i-rows for n particles (`wgs` times the i per lane
+// for the forward), at most 64 (one wave per i).  Timed at n = 16384 over
+// njs = 8..64: the forward is fastest at 1024 workgroups (FP32 with two i
+// per lane: 215 µs, against 221 µs with one i per lane at its best, 2048
+// workgroups), the adjoint at 512 (FP64 805 against 908 µs at 2048): more
+// lanes per i mean a longer cross-lane fold at the end and fewer rows per
+// workgroup, which costs the heavier kernels more than the extra waves
+// gain.
 inline int lanes_per_i(int n, long long wgs)
 {
     int njs = 8;
@@ -315,13 +335,19 @@ inline int lanes_per_i(int n, long long wgs)
     return njs;
 }
 
+// FP32: two i per lane; FP64: one (twice the registers would cost the FP64
+// kernel half its waves, and it issues near the FP64 rate already)
+template <class T>
+constexpr int kForceIPL = sizeof(T) == 4 ? 2 : 1;
+
 template <class T, int NJS>
 void launch_forces_t(int n, const T *x, const T *v, const T *m, T eps2, T *acc, T *jerk,
                      hipStream_t st)
 {
-    const int grid = (n + kWG / NJS - 1) / (kWG / NJS);
-    hipLaunchKernelGGL((k_forces<T, NJS>), dim3(grid), dim3(kWG), 0, st, n, x, v, m, eps2, acc,
-                       jerk);
+    constexpr int rows = (kWG / NJS) * kForceIPL<T>;
+    const int grid = (n + rows - 1) / rows;
+    hipLaunchKernelGGL((k_forces<T, NJS, kForceIPL<T>>), dim3(grid), dim3(kWG), 0, st, n, x, v,
+                       m, eps2, acc, jerk);
 }
 
 template <class T>
@@ -329,7 +355,7 @@ int launch_forces(int n, const T *x, const T *v, const T *m, T eps2, T *acc, T *
                   hipStream_t st)
 {
     if (n <= 0) return 0;
-    switch (lanes_per_i(n, sizeof(T) == 4 ? 2048 : 1024)) {
+    switch (lanes_per_i(n, 1024ll * kForceIPL<T>)) {
     case 8: launch_forces_t<T, 8>(n, x, v, m, eps2, acc, jerk, st); break;
     case 16: launch_forces_t<T, 16>(n, x, v, m, eps2, acc, jerk, st); break;
     case 32: launch_forces_t<T, 32>(n, x, v, m, eps2, acc, jerk, st); break;
