@@ -2,9 +2,11 @@
 Hermite integrator (amuse_amd/diff.py).
 
 * forward kernels: FP64 within 1e-15 sum|term| of the 80-bit oracle, FP32
-  within 2e-6 sum|term|;
+  within 2e-6 sum|term|, at n = 2000 and 16384 (every lanes-per-i form the
+  sizes select, one and two i per lane);
 * adjoint kernel: equal to torch autograd through a plain float64 torch
-  restatement of the pair sum (1e-11 relative), for x, v and m;
+  restatement of the pair sum (1e-11 relative), for x, v and m, at
+  n = 96 .. 16384 (64 down to 8 lanes per i);
 * whole pipeline (integrate + Lagrangian-radius loss) in FP64: gradients
   match central finite differences along random directions;
 * the fused Hermite step (predictor, pair sum and corrector kernels, with
@@ -33,17 +35,31 @@ def tensors(n, seed, dtype=torch.float64, eps_scale=1.0):
             torch.tensor(m, dtype=dtype, device=dev))
 
 
-def torch_pair_force(x, v, m, eps2):
-    r = x[None, :, :] - x[:, None, :]
-    w = v[None, :, :] - v[:, None, :]
+def _pair_rows(x, v, m, eps2, i0, i1):
+    """acc, jerk of particles i0..i1-1 as plain torch operations."""
+    r = x[None, :, :] - x[i0:i1, None, :]
+    w = v[None, :, :] - v[i0:i1, None, :]
     s = (r * r).sum(-1) + eps2
-    eye = torch.eye(x.shape[0], dtype=torch.bool, device=x.device)
-    y = torch.where(eye, torch.zeros_like(s), torch.rsqrt(s))
+    self_pair = torch.arange(x.shape[0], device=x.device)[None, :] == \
+        torch.arange(i0, i1, device=x.device)[:, None]
+    y = torch.where(self_pair, torch.zeros_like(s), torch.rsqrt(s))
     y3, y5 = y ** 3, y ** 5
     rv = (r * w).sum(-1)
     acc = (m[None, :, None] * r * y3[..., None]).sum(1)
     jerk = (m[None, :, None] * (w * y3[..., None] - 3 * rv[..., None] * r * y5[..., None])).sum(1)
     return acc, jerk
+
+
+def torch_pair_force(x, v, m, eps2, rows=2048):
+    """The FP64 torch restatement of the pair sum; above `rows` particles in
+    checkpointed blocks of rows (the saved tensors of one block at a time)."""
+    from torch.utils.checkpoint import checkpoint
+    n = x.shape[0]
+    if n <= rows:
+        return _pair_rows(x, v, m, eps2, 0, n)
+    parts = [checkpoint(_pair_rows, x, v, m, eps2, i0, min(n, i0 + rows), use_reentrant=False)
+             for i0 in range(0, n, rows)]
+    return torch.cat([p[0] for p in parts]), torch.cat([p[1] for p in parts])
 
 
 def test_fused_step_matches_torch_step():
@@ -72,26 +88,35 @@ def test_fused_step_matches_torch_step():
             assert err <= 1e-11, (name, which, err)
 
 
-def test_forward_kernels_vs_oracle():
+@pytest.mark.parametrize("n", [2000, 16384])
+def test_forward_kernels_vs_oracle(n):
     """FP64 against the oracle on the same inputs; FP32 against the oracle on
-    the float-rounded inputs (the kernel's arithmetic, not input rounding)."""
+    the float-rounded inputs (the kernel's arithmetic, not input rounding).
+    n = 2000 runs 64 lanes per i; n = 16384 16 (FP64) and 32 with two i per
+    lane (FP32), checked on 512 sampled i."""
     from amuse_amd.diff import pair_force
-    x, v, m = tensors(2000, 1)
-    ids = np.arange(2000, dtype=np.int32)
+    from conftest import par_oracle
+    x, v, m = tensors(n, 1)
+    ids = np.arange(n, dtype=np.int32)
+    sel = np.arange(n) if n <= 2000 else np.arange(0, n, n // 512)
     for dtype, tols in ((torch.float64, (1e-15, 4e-15)), (torch.float32, (2e-6, 1e-5))):
         xx, vv, mm = x.to(dtype), v.to(dtype), m.to(dtype)
         a, j = pair_force(xx, vv, mm, 1e-4)
         xn, vn, mn = (t.double().cpu().numpy() for t in (xx, vv, mm))
-        ea, ej, _, sa, sj, _ = oracle.force_ld(ids, xn, vn, 1e-4, ids, mn, xn, vn)
-        for val, ex, s, tol, name in ((a, ea, sa, tols[0], "acc"), (j, ej, sj, tols[1], "jerk")):
+        ea, ej, _, sa, sj, _ = par_oracle(oracle.force_ld, ids[sel], xn[sel], vn[sel], 1e-4, ids,
+                                          mn, xn, vn)
+        for val, ex, s, tol, name in ((a[sel], ea, sa, tols[0], "acc"), (j[sel], ej, sj, tols[1], "jerk")):
             d = np.abs(val.double().cpu().numpy() - ex).max(1)
             print(dtype, name, "max |d|/sum|term|", (d / s).max(), flush=True)
             assert (d <= tol * s).all(), (dtype, name, (d / s).max())
 
 
-def test_adjoint_matches_autograd_reference():
+@pytest.mark.parametrize("n", [96, 4096, 8192, 16384])
+def test_adjoint_matches_autograd_reference(n):
+    """Against torch autograd of the FP64 restatement: n = 96, 4096, 8192
+    and 16384 run the adjoint with 64, 32, 16 and 8 lanes per i."""
     from amuse_amd.diff import pair_force
-    x, v, m = tensors(96, 2)
+    x, v, m = tensors(n, 2)
     gen = torch.Generator(device="cuda").manual_seed(3)
     ga = torch.randn(x.shape, dtype=x.dtype, device="cuda", generator=gen)
     gj = torch.randn(x.shape, dtype=x.dtype, device="cuda", generator=gen)
