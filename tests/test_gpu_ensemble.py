@@ -81,6 +81,14 @@ def test_ensemble_forces():
             d = np.abs(val - ex).reshape(len(s), -1).max(1)
             print(f"cluster n={n} {name}: max |d|/sum|term| {(d / s).max():.3e}", flush=True)
             assert (d <= tol * s).all(), (n, name, (d / s).max())
+        # Newton's third law within each cluster: sum m a = sum m jerk = 0
+        mm = m[sl][:, None]
+        # (measured: FP64 <= 1.1e-16, mixed <= 4.6e-8 of the sums of magnitudes)
+        for val, tol, name in ((a64[sl], 1e-14, "acc64"), (j64[sl], 1e-14, "jerk64"),
+                               (amx[sl], 1e-6, "accmx"), (jmx[sl], 1e-6, "jerkmx")):
+            r = np.linalg.norm((mm * val).sum(0)) / (mm[:, 0] * np.linalg.norm(val, axis=1)).sum()
+            print(f"cluster n={n} {name}: |sum m q|/sum|m q| {r:.2e}", flush=True)
+            assert r <= tol, (n, name, r)
     ens.close()
 
 
