@@ -684,6 +684,10 @@ def main() -> int:
                     help="full: full-N force pass at N=2^20 (default); blocks: i-blocks of "
                          "2^k against N=65536 with per-j times, FP64 vs mixed errors")
     args = ap.parse_args()
+    if args.gpus != 1:
+        print("bench.py measures one GPU: the force path is built and measured on a single "
+              "MI355X (multi-GPU j-domain sharding is a design note only)", file=sys.stderr)
+        return 2
     if args.copy_calib:
         return copy_calibration()
     if args.workload == "blocks":
@@ -702,10 +706,6 @@ def main() -> int:
         return worker_study(args)
     if args.workload == "c4evolve":
         return c4_evolve(args)
-    if args.gpus != 1:
-        print("bench.py measures one GPU: the force path is built and measured on a single "
-              "MI355X (multi-GPU j-domain sharding is a design note only)", file=sys.stderr)
-        return 2
 
     from amuse_amd.g6 import G6
     n, eps2 = args.n, args.eps2
